@@ -1,0 +1,886 @@
+// C ABI of libfocus_gpu.so (include/focus_gpu.h): contexts, the stack driver (pass 1 / pass 2 /
+// collapse, src/stack/pipeline.zig:361-605 and src/fuse/pipeline.zig:254-445), the frame cache
+// under the memory budget, and the host-side pose preparation (src/optimize.zig:3555-3586).
+#include <hip/hip_runtime.h>
+
+#include <math.h>
+#include <string.h>
+
+#include <mutex>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/focus_gpu.h"
+#include "fg_internal.h"
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(int code, const std::string& msg) {
+    g_last_error = msg;
+    return code;
+}
+int fail_hip(hipError_t e, const char* what) {
+    g_last_error = std::string(what) + ": " + hipGetErrorString(e);
+    return e == hipErrorOutOfMemory ? FG_ERR_OOM_DEVICE : FG_ERR_DEVICE;
+}
+
+#define HIP_TRY(expr, what)                      \
+    do {                                         \
+        hipError_t e_ = (expr);                  \
+        if (e_ != hipSuccess) return fail_hip(e_, what); \
+    } while (0)
+
+// ---- pose math (host, f64) ----
+const double kPi = 3.14159265358979323846;
+
+double focal_length_px(uint32_t width, double hfov) {  // focalLengthPixels src/optimize.zig:3705-3708
+    double h = hfov < 1e-3 ? 1e-3 : (hfov > 179.0 ? 179.0 : hfov);
+    return ((double)width * 0.5) / tan(h * kPi / 360.0);
+}
+
+void mat_mul(const double a[3][3], const double b[3][3], double out[3][3]) {  // matrixMatrixMul :3951-3959
+    double r[3][3];
+    for (int i = 0; i < 3; i++)
+        for (int k = 0; k < 3; k++) r[i][k] = a[i][0] * b[0][k] + a[i][1] * b[1][k] + a[i][2] * b[2][k];
+    memcpy(out, r, sizeof r);
+}
+
+void set_matrix(double a, double b, double c, double m[3][3]) {  // setMatrix(a, b, c, cl=true) :3901-3921
+    double mx[3][3] = {{1, 0, 0}, {0, cos(a), sin(a)}, {0, -sin(a), cos(a)}};
+    double my[3][3] = {{cos(b), 0, -sin(b)}, {0, 1, 0}, {sin(b), 0, cos(b)}};
+    double mz[3][3] = {{cos(c), sin(c), 0}, {-sin(c), cos(c), 0}, {0, 0, 1}};
+    double d[3][3];
+    mat_mul(mz, mx, d);
+    mat_mul(d, my, m);
+}
+
+bool has_translation(const fg_pose& p) {  // hasTranslation :3962-3964
+    return fabs(p.trans_x) > 1e-12 || fabs(p.trans_y) > 1e-12 || fabs(p.trans_z) > 1e-12;
+}
+bool basic_rectilinear(const fg_pose& p) {  // hasBasicRectilinearPose :3074-3081
+    return !has_translation(p) && fabs(p.radial_a) <= 1e-12 && fabs(p.radial_b) <= 1e-12 &&
+           fabs(p.radial_c) <= 1e-12 && fabs(p.center_shift_x) <= 1e-12 && fabs(p.center_shift_y) <= 1e-12;
+}
+
+}  // namespace
+
+namespace fg {
+
+void make_levels(uint32_t w, uint32_t h, uint32_t max_levels, LevelTable* t) {  // computeLevelCount :163-176
+    t->n = 0;
+    size_t off = 0;
+    for (;;) {
+        if (t->n >= kMaxLevels) break;
+        t->lv[t->n] = Level{w, h, off};
+        t->n++;
+        off += (size_t)w * h;
+        if ((w == 1 && h == 1) || (max_levels && (uint32_t)t->n >= max_levels)) break;
+        w = w > 1 ? (w + 1) / 2 : 1;
+        h = h > 1 ? (h + 1) / 2 : 1;
+    }
+    t->total = off;
+}
+
+// initInverseTransformCache src/optimize.zig:3555-3586, folded into one 3x3 matrix per frame.
+// Basic rectilinear poses: the cached legacy chain (:3618-3634) equals the pinhole homography
+// src = c + f * (A p).xy / (A p).z with A = pitch_roll^T * yawMatrix(yaw), p = (x-cx, y-cy, d);
+// it is written here in the lens path's form local = B (x-cx, y-cy, -d), B = D A D, D = diag(1,1,-1).
+void make_remap_params(const double pose15[15], uint32_t w, uint32_t h, int32_t roi_left, int32_t roi_top,
+                       RemapParams* p) {
+    fg_pose pose;
+    memcpy(&pose, pose15, sizeof pose);
+    memset(p, 0, sizeof *p);
+    double cx = ((double)w - 1.0) * 0.5, cy = ((double)h - 1.0) * 0.5;
+    double pano = focal_length_px(w, pose.base_hfov_degrees);
+    double eff = pose.base_hfov_degrees + pose.hfov_delta;
+    double img_f = focal_length_px(w, eff < 1e-3 ? 1e-3 : (eff > 179.0 ? 179.0 : eff));
+    double pr[3][3], ym[3][3], w2l[3][3];
+    set_matrix(-pose.pitch, 0.0, pose.roll, pr);
+    double cyw = cos(-pose.yaw), syw = sin(-pose.yaw);
+    double ymt[3][3] = {{cyw, 0, -syw}, {0, 1, 0}, {syw, 0, cyw}};
+    memcpy(ym, ymt, sizeof ym);
+    mat_mul(pr, ym, w2l);
+    p->neg_d = -pano;
+    p->dcx = cx;
+    p->dcy = cy;
+    p->scx = cx;
+    p->scy = cy;
+    p->focal = img_f;
+    p->roi_left = roi_left;
+    p->roi_top = roi_top;
+    double hw = ((double)w - 1.0) * 0.5, hh = ((double)h - 1.0) * 0.5;
+    double den = hw * hw + hh * hh;
+    p->inv_norm = den >= 1e-12 ? 1.0 / den : 1.0;  // radialNormalizer :4150-4156
+    p->ra = pose.radial_a;
+    p->rb = pose.radial_b;
+    p->rc = pose.radial_c;
+    if (basic_rectilinear(pose)) {
+        double c = cos(pose.yaw), s = sin(pose.yaw);
+        double ry[3][3] = {{c, 0, -s}, {0, 1, 0}, {s, 0, c}};
+        double prt[3][3], a[3][3];
+        for (int i = 0; i < 3; i++)
+            for (int k = 0; k < 3; k++) prt[i][k] = pr[k][i];
+        mat_mul(prt, ry, a);
+        const double d[3] = {1.0, 1.0, -1.0};
+        for (int i = 0; i < 3; i++)
+            for (int k = 0; k < 3; k++) p->B[i * 3 + k] = d[i] * a[i][k] * d[k];
+        p->mode = REMAP_HOMOGRAPHY;
+        p->legacy = 1;
+        return;
+    }
+    for (int i = 0; i < 3; i++)
+        for (int k = 0; k < 3; k++) p->B[i * 3 + k] = w2l[i][k];
+    p->csx = pose.center_shift_x;
+    p->csy = pose.center_shift_y;
+    if (has_translation(pose)) {
+        // translationPlaneNormal :3966-3968 = cartErect(tp_yaw, -tp_pitch, 1) :4045-4053
+        double phi = pose.translation_plane_yaw / 1.0;
+        double tz = kPi * 0.5 - (-pose.translation_plane_pitch / 1.0);
+        p->pn[0] = sin(tz) * sin(phi);
+        p->pn[1] = cos(tz);
+        p->pn[2] = sin(tz) * -cos(phi);
+        p->tx = pose.trans_x;
+        p->ty = pose.trans_y;
+        p->tz = pose.trans_z;
+        p->mode = REMAP_TRANSLATION;
+        return;
+    }
+    p->mode = REMAP_LENS;
+    p->radial = (pose.radial_a != 0.0 || pose.radial_b != 0.0 || pose.radial_c != 0.0) ? 1 : 0;
+}
+
+}  // namespace fg
+
+// ---------------------------------------------------------------------------
+
+struct fg_context {
+    int device;
+    float memory_fraction;
+    hipStream_t stream;
+};
+
+struct fg_stack {
+    fg_context* ctx;
+    fg_stack_opts opts;
+    hipStream_t stream;
+    bool own_stream;
+    uint32_t W, H;
+    int sb;
+    fg::LevelTable lt;
+    bool cached;
+    uint32_t added, pass2_done;
+    bool finished;
+    // device memory
+    std::vector<void*> allocs;
+    uint8_t* frames;      // cached RGBA frames (or one slot)
+    float* weights;       // cached weights (or one slot)
+    float* S;
+    uint8_t* U;
+    float* acc;
+    float4* G;
+    float2* M;
+    float* best;
+    uint8_t* hard;
+    uint8_t* soft;
+    uint8_t* src_stage;
+    size_t src_stage_bytes;
+    float4* dbgG;
+    float2* dbgM;
+    float* accDbg;
+    uint8_t* out_dev;
+    float* merge_gs;
+    float* merge_gh;
+};
+
+namespace {
+
+size_t frame_bytes(const fg_stack* s) { return (size_t)s->W * s->H * 4 * s->sb; }
+size_t weight_bytes(const fg_stack* s) { return (size_t)s->W * s->H * sizeof(float); }
+
+int dev_alloc(fg_stack* s, size_t bytes, void** out) {
+    void* p = nullptr;
+    hipError_t e = hipMalloc(&p, bytes ? bytes : 256);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        return fail(FG_ERR_OOM_DEVICE, "hipMalloc of " + std::to_string(bytes) + " bytes failed");
+    }
+    s->allocs.push_back(p);
+    *out = p;
+    return FG_OK;
+}
+
+void free_stack(fg_stack* s) {
+    if (!s) return;
+    if (s->stream) (void)hipStreamSynchronize(s->stream);
+    for (void* p : s->allocs) (void)hipFree(p);
+    if (s->own_stream && s->stream) (void)hipStreamDestroy(s->stream);
+    delete s;
+}
+
+int check_desc(const fg_image_desc* d) {
+    if (!d) return fail(FG_ERR_INVALID_ARG, "null image descriptor");
+    if (d->width == 0 || d->height == 0) return fail(FG_ERR_INVALID_ARG, "empty image");
+    if (d->color_channels != 1 && d->color_channels != 3)
+        return fail(FG_ERR_UNSUPPORTED, "color_channels must be 1 or 3");
+    if (d->extra_channels > 1) return fail(FG_ERR_UNSUPPORTED, "extra_channels must be 0 or 1");
+    if (d->sample_type != FG_SAMPLE_U8 && d->sample_type != FG_SAMPLE_U16)
+        return fail(FG_ERR_UNSUPPORTED, "sample_type must be u8 or u16");
+    return FG_OK;
+}
+
+size_t desc_bytes(const fg_image_desc* d) {
+    return (size_t)d->width * d->height * (d->color_channels + d->extra_channels) *
+           (d->sample_type == FG_SAMPLE_U16 ? 2 : 1);
+}
+
+int set_device(int dev) {
+    HIP_TRY(hipSetDevice(dev), "hipSetDevice");
+    return FG_OK;
+}
+
+// Brings one frame (source + pose, or an aligned frame) into the RGBA slot `dst` on the stack's stream.
+int ingest_frame(fg_stack* s, const fg_image_desc* d, const void* px, bool device_ptr, const fg_pose* pose,
+                 const fg_rect* roi, uint8_t* dst) {
+    int rc = check_desc(d);
+    if (rc) return rc;
+    if (!px) return fail(FG_ERR_INVALID_ARG, "null pixel pointer");
+    int sb = d->sample_type == FG_SAMPLE_U16 ? 2 : 1;
+    if (sb != s->sb) return fail(FG_ERR_FORMAT_MISMATCH, "frame sample type differs from the stack's");
+    if (d->color_channels != 3) return fail(FG_ERR_UNSUPPORTED, "pyramid fusion needs rgb frames");
+    const void* dsrc = px;
+    if (!device_ptr) {
+        size_t nb = desc_bytes(d);
+        if (nb > s->src_stage_bytes) {
+            void* p = nullptr;
+            rc = dev_alloc(s, nb, &p);
+            if (rc) return rc;
+            s->src_stage = (uint8_t*)p;
+            s->src_stage_bytes = nb;
+        }
+        HIP_TRY(hipMemcpyAsync(s->src_stage, px, nb, hipMemcpyHostToDevice, s->stream), "upload frame");
+        HIP_TRY(hipStreamSynchronize(s->stream), "upload frame");
+        dsrc = s->src_stage;
+    }
+    if (pose) {
+        if (d->extra_channels != 0) return fail(FG_ERR_UNSUPPORTED, "remap sources carry no alpha channel");
+        fg_rect r = roi ? *roi : fg_rect{0, 0, (int32_t)d->width, (int32_t)d->height};
+        if ((uint32_t)(r.right - r.left) != s->W || (uint32_t)(r.bottom - r.top) != s->H || r.right <= r.left ||
+            r.bottom <= r.top)
+            return fail(FG_ERR_SIZE_MISMATCH, "roi size differs from the stack size");
+        fg::RemapParams p;
+        fg::make_remap_params((const double*)pose, d->width, d->height, r.left, r.top, &p);
+        HIP_TRY(fg::launch_remap(dsrc, sb, 3, d->width, d->height, p, s->W, s->H, dst, s->stream), "remap kernel");
+    } else {
+        if (d->width != s->W || d->height != s->H)
+            return fail(FG_ERR_SIZE_MISMATCH, "aligned frame size differs from the stack size");
+        HIP_TRY(fg::launch_to_rgba(dsrc, sb, (int)d->extra_channels, s->W, s->H, dst, s->stream), "to_rgba kernel");
+    }
+    return FG_OK;
+}
+
+int run_pass2(fg_stack* s, uint32_t k, const uint8_t* rgba, const float* w) {
+    fg::Pass2Args a{rgba, s->sb, w, s->S, s->U, s->opts.frame_count, &s->lt, s->G, s->M, s->acc};
+    HIP_TRY(fg::launch_pass2_frame(a, s->stream), "pass2 kernels");
+    if (s->opts.debug_frame >= 0 && (uint32_t)s->opts.debug_frame == k) {
+        HIP_TRY(fg::launch_level0_debug(a, s->dbgG, s->dbgM, s->stream), "debug level0");
+        size_t rest = s->lt.total - s->lt.lv[0].w * (size_t)s->lt.lv[0].h;
+        size_t off = s->lt.lv[0].w * (size_t)s->lt.lv[0].h;
+        if (rest) {
+            HIP_TRY(hipMemcpyAsync(s->dbgG + off, s->G + off, rest * sizeof(float4), hipMemcpyDeviceToDevice,
+                                   s->stream),
+                    "debug copy");
+            HIP_TRY(hipMemcpyAsync(s->dbgM + off, s->M + off, rest * sizeof(float2), hipMemcpyDeviceToDevice,
+                                   s->stream),
+                    "debug copy");
+        }
+    }
+    return FG_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* fg_last_error(void) { return g_last_error.c_str(); }
+const char* fg_version(void) { return "focus_gpu 0.1 (gfx950)"; }
+
+int fg_context_create(const fg_context_opts* opts, fg_context** out) {
+    if (!out) return fail(FG_ERR_INVALID_ARG, "null out");
+    *out = nullptr;
+    int dev = opts ? opts->device : 0;
+    int count = 0;
+    hipError_t e = hipGetDeviceCount(&count);
+    if (e != hipSuccess || count == 0) return fail(FG_ERR_DEVICE, "no HIP device available");
+    if (dev < 0 || dev >= count) return fail(FG_ERR_INVALID_ARG, "device index out of range");
+    int rc = set_device(dev);
+    if (rc) return rc;
+    fg_context* c = new (std::nothrow) fg_context();
+    if (!c) return fail(FG_ERR_OOM_HOST, "context allocation");
+    c->device = dev;
+    c->memory_fraction = opts ? opts->memory_fraction : 0.5f;
+    e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        delete c;
+        return fail_hip(e, "hipStreamCreate");
+    }
+    *out = c;
+    return FG_OK;
+}
+
+void fg_context_destroy(fg_context* ctx) {
+    if (!ctx) return;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
+int fg_remap_device(fg_context* ctx, const fg_image_desc* src, const void* d_src, const fg_pose* pose,
+                    const fg_rect* roi, void* d_dst, void* stream) {
+    if (!ctx || !pose || !d_src || !d_dst) return fail(FG_ERR_INVALID_ARG, "null argument");
+    int rc = check_desc(src);
+    if (rc) return rc;
+    if (src->extra_channels != 0) return fail(FG_ERR_UNSUPPORTED, "remap sources carry no alpha channel");
+    rc = set_device(ctx->device);
+    if (rc) return rc;
+    fg_rect r = roi ? *roi : fg_rect{0, 0, (int32_t)src->width, (int32_t)src->height};
+    if (r.right <= r.left || r.bottom <= r.top) return fail(FG_ERR_INVALID_ARG, "empty roi");
+    fg::RemapParams p;
+    fg::make_remap_params((const double*)pose, src->width, src->height, r.left, r.top, &p);
+    hipStream_t s = stream ? (hipStream_t)stream : ctx->stream;
+    HIP_TRY(fg::launch_remap(d_src, src->sample_type == FG_SAMPLE_U16 ? 2 : 1, (int)src->color_channels, src->width,
+                             src->height, p, (uint32_t)(r.right - r.left), (uint32_t)(r.bottom - r.top), d_dst, s),
+            "remap kernel");
+    return FG_OK;
+}
+
+int fg_remap(fg_context* ctx, const fg_image_desc* src, const void* src_px, const fg_pose* pose, const fg_rect* roi,
+             void* dst_px) {
+    if (!ctx || !pose || !src_px || !dst_px) return fail(FG_ERR_INVALID_ARG, "null argument");
+    int rc = check_desc(src);
+    if (rc) return rc;
+    rc = set_device(ctx->device);
+    if (rc) return rc;
+    fg_rect r = roi ? *roi : fg_rect{0, 0, (int32_t)src->width, (int32_t)src->height};
+    if (r.right <= r.left || r.bottom <= r.top) return fail(FG_ERR_INVALID_ARG, "empty roi");
+    size_t sb = src->sample_type == FG_SAMPLE_U16 ? 2 : 1;
+    size_t in_b = desc_bytes(src);
+    size_t out_b = (size_t)(r.right - r.left) * (r.bottom - r.top) * (src->color_channels + 1) * sb;
+    void *din = nullptr, *dout = nullptr;
+    hipError_t e = hipMalloc(&din, in_b);
+    if (e != hipSuccess) return fail_hip(e, "hipMalloc");
+    e = hipMalloc(&dout, out_b);
+    if (e != hipSuccess) {
+        (void)hipFree(din);
+        return fail_hip(e, "hipMalloc");
+    }
+    rc = FG_OK;
+    e = hipMemcpyAsync(din, src_px, in_b, hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess) {
+        rc = fg_remap_device(ctx, src, din, pose, &r, dout, ctx->stream);
+        if (rc == FG_OK) {
+            e = hipMemcpyAsync(dst_px, dout, out_b, hipMemcpyDeviceToHost, ctx->stream);
+            if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+        }
+    }
+    (void)hipFree(din);
+    (void)hipFree(dout);
+    if (rc) return rc;
+    if (e != hipSuccess) return fail_hip(e, "fg_remap");
+    return FG_OK;
+}
+
+int fg_stack_create(fg_context* ctx, const fg_stack_opts* o, fg_stack** out) {
+    if (!ctx || !o || !out) return fail(FG_ERR_INVALID_ARG, "null argument");
+    *out = nullptr;
+    if (o->method != FG_METHOD_PYRAMID_CONTRAST && o->method != FG_METHOD_HYBRID_PYRAMID_CONTRAST)
+        return fail(FG_ERR_UNSUPPORTED, "method must be pyramid-contrast or hybrid-pyramid-contrast");
+    if (o->contrast_window_size < 3 || (o->contrast_window_size & 1) == 0 || o->contrast_window_size > 31)
+        return fail(FG_ERR_INVALID_ARG, "contrast_window_size must be odd and in [3, 31]");
+    if (!(o->hybrid_sharpness >= 0.0f && o->hybrid_sharpness <= 1.0f))
+        return fail(FG_ERR_INVALID_ARG, "hybrid_sharpness must be in [0, 1]");
+    if (o->frame_count == 0 || o->width == 0 || o->height == 0) return fail(FG_ERR_INVALID_ARG, "empty stack");
+    if (o->sample_type != FG_SAMPLE_U8 && o->sample_type != FG_SAMPLE_U16)
+        return fail(FG_ERR_UNSUPPORTED, "sample_type must be u8 or u16");
+    int rc = set_device(ctx->device);
+    if (rc) return rc;
+    fg_stack* s = new (std::nothrow) fg_stack();
+    if (!s) return fail(FG_ERR_OOM_HOST, "stack allocation");
+    s->ctx = ctx;
+    s->opts = *o;
+    s->W = o->width;
+    s->H = o->height;
+    s->sb = o->sample_type == FG_SAMPLE_U16 ? 2 : 1;
+    fg::make_levels(s->W, s->H, o->max_levels, &s->lt);
+    if (o->stream) {
+        s->stream = (hipStream_t)o->stream;
+        s->own_stream = false;
+    } else {
+        hipError_t e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
+        if (e != hipSuccess) {
+            delete s;
+            return fail_hip(e, "hipStreamCreate");
+        }
+        s->own_stream = true;
+    }
+    const bool hybrid = o->method == FG_METHOD_HYBRID_PYRAMID_CONTRAST;
+    const size_t px = (size_t)s->W * s->H;
+    const size_t tot = s->lt.total;
+    size_t fixed = px * 4 + px + tot * 12 + tot * 16 + tot * 8 + px * 3 * s->sb;
+    if (hybrid) fixed += px * 4 + 2 * px * 3 * s->sb + 2 * tot * 12;
+    if (o->debug_frame >= 0) fixed += tot * 36;
+    size_t per_frame = frame_bytes(s) + weight_bytes(s);
+    size_t free_b = 0, total_b = 0;
+    (void)hipMemGetInfo(&free_b, &total_b);
+    float frac = o->memory_fraction >= 0.0f ? o->memory_fraction : ctx->memory_fraction;
+    double budget = frac > 0.0f ? (double)frac * (double)free_b : (double)free_b - 1.0e9;
+    s->cached = (double)fixed + (double)o->frame_count * (double)per_frame <= budget;
+    size_t slots = s->cached ? o->frame_count : 1;
+    void* p;
+#define ALLOC(field, bytes)                      \
+    do {                                         \
+        rc = dev_alloc(s, (bytes), &p);          \
+        if (rc) {                                \
+            free_stack(s);                       \
+            return rc;                           \
+        }                                        \
+        s->field = (decltype(s->field))p;        \
+    } while (0)
+    ALLOC(frames, slots * frame_bytes(s));
+    ALLOC(weights, slots * weight_bytes(s));
+    ALLOC(S, px * sizeof(float));
+    ALLOC(U, px);
+    ALLOC(acc, tot * 3 * sizeof(float));
+    ALLOC(G, tot * sizeof(float4));
+    ALLOC(M, tot * sizeof(float2));
+    ALLOC(out_dev, px * 3 * s->sb);
+    if (hybrid) {
+        ALLOC(best, px * sizeof(float));
+        ALLOC(hard, px * 3 * s->sb);
+        ALLOC(soft, px * 3 * s->sb);
+        ALLOC(merge_gs, tot * 3 * sizeof(float));
+        ALLOC(merge_gh, tot * 3 * sizeof(float));
+    }
+    if (o->debug_frame >= 0) {
+        ALLOC(dbgG, tot * sizeof(float4));
+        ALLOC(dbgM, tot * sizeof(float2));
+        ALLOC(accDbg, tot * 3 * sizeof(float));
+    }
+#undef ALLOC
+    hipError_t e = fg::launch_zero(s->S, px * sizeof(float), s->stream);
+    if (e == hipSuccess) e = fg::launch_zero(s->U, px, s->stream);
+    if (e == hipSuccess) e = fg::launch_zero(s->acc, tot * 3 * sizeof(float), s->stream);
+    if (e == hipSuccess && hybrid) e = fg::launch_fill_f32(s->best, px, -INFINITY, s->stream);
+    if (e != hipSuccess) {
+        rc = fail_hip(e, "stack init");
+        free_stack(s);
+        return rc;
+    }
+    *out = s;
+    return FG_OK;
+}
+
+void fg_stack_destroy(fg_stack* s) {
+    if (!s) return;
+    (void)hipSetDevice(s->ctx->device);
+    free_stack(s);
+}
+
+static int add_frame(fg_stack* s, const fg_image_desc* d, const void* px, bool dev, const fg_pose* pose,
+                     const fg_rect* roi) {
+    if (!s) return fail(FG_ERR_INVALID_ARG, "null stack");
+    if (s->finished || s->pass2_done) return fail(FG_ERR_STATE, "pass 1 is closed");
+    if (s->added >= s->opts.frame_count) return fail(FG_ERR_STATE, "more frames than frame_count");
+    int rc = set_device(s->ctx->device);
+    if (rc) return rc;
+    const uint32_t k = s->added;
+    uint8_t* slot = s->frames + (s->cached ? k : 0) * frame_bytes(s);
+    float* w = s->weights + (s->cached ? k : 0) * (size_t)s->W * s->H;
+    rc = ingest_frame(s, d, px, dev, pose, roi, slot);
+    if (rc) return rc;
+    fg::Pass1Args a{slot, s->sb, s->W, s->H, s->opts.contrast_window_size, w, s->S, s->U, s->best, s->hard};
+    HIP_TRY(fg::launch_pass1(a, s->stream), "pass1 kernel");
+    s->added++;
+    return FG_OK;
+}
+
+int fg_stack_add_frame(fg_stack* s, const fg_image_desc* d, const void* px, const fg_pose* pose, const fg_rect* roi) {
+    return add_frame(s, d, px, false, pose, roi);
+}
+int fg_stack_add_frame_device(fg_stack* s, const fg_image_desc* d, const void* px, const fg_pose* pose,
+                              const fg_rect* roi) {
+    return add_frame(s, d, px, true, pose, roi);
+}
+
+int fg_stack_needs_pass2_frames(const fg_stack* s) { return s && !s->cached ? 1 : 0; }
+
+static int pass2_frame(fg_stack* s, const fg_image_desc* d, const void* px, bool dev, const fg_pose* pose,
+                       const fg_rect* roi) {
+    if (!s) return fail(FG_ERR_INVALID_ARG, "null stack");
+    if (s->cached) return fail(FG_ERR_STATE, "frames are cached on the device; call fg_stack_finish");
+    if (s->finished) return fail(FG_ERR_STATE, "stack already finished");
+    if (s->added != s->opts.frame_count) return fail(FG_ERR_STATE, "pass 1 incomplete");
+    if (s->pass2_done >= s->opts.frame_count) return fail(FG_ERR_STATE, "more pass-2 frames than frame_count");
+    int rc = set_device(s->ctx->device);
+    if (rc) return rc;
+    rc = ingest_frame(s, d, px, dev, pose, roi, s->frames);
+    if (rc) return rc;
+    // The weight map is recomputed as in pass 1 (computeWeightMapForRemapped + applySupport,
+    // src/stack/pipeline.zig:571-572); S and U are final, so they are left untouched.
+    fg::Pass1Args a{s->frames, s->sb, s->W, s->H, s->opts.contrast_window_size, s->weights, nullptr, nullptr,
+                    nullptr, nullptr};
+    HIP_TRY(fg::launch_pass1(a, s->stream), "pass1 kernel");
+    rc = run_pass2(s, s->pass2_done, s->frames, s->weights);
+    if (rc) return rc;
+    s->pass2_done++;
+    return FG_OK;
+}
+
+int fg_stack_pass2_frame(fg_stack* s, const fg_image_desc* d, const void* px, const fg_pose* pose, const fg_rect* roi) {
+    return pass2_frame(s, d, px, false, pose, roi);
+}
+int fg_stack_pass2_frame_device(fg_stack* s, const fg_image_desc* d, const void* px, const fg_pose* pose,
+                                const fg_rect* roi) {
+    return pass2_frame(s, d, px, true, pose, roi);
+}
+
+static int finish(fg_stack* s) {
+    if (!s) return fail(FG_ERR_INVALID_ARG, "null stack");
+    if (s->finished) return FG_OK;
+    if (s->added != s->opts.frame_count) return fail(FG_ERR_STATE, "not every frame went through pass 1");
+    int rc = set_device(s->ctx->device);
+    if (rc) return rc;
+    if (s->cached) {
+        for (uint32_t k = 0; k < s->opts.frame_count; k++) {
+            rc = run_pass2(s, k, s->frames + k * frame_bytes(s), s->weights + (size_t)k * s->W * s->H);
+            if (rc) return rc;
+        }
+    } else if (s->pass2_done != s->opts.frame_count) {
+        return fail(FG_ERR_STATE, "uncached stack: feed every frame through fg_stack_pass2_frame first");
+    }
+    if (s->accDbg)
+        HIP_TRY(hipMemcpyAsync(s->accDbg, s->acc, s->lt.total * 3 * sizeof(float), hipMemcpyDeviceToDevice, s->stream),
+                "debug copy");
+    HIP_TRY(fg::launch_collapse(s->acc, &s->lt, s->stream), "collapse");
+    const bool hybrid = s->opts.method == FG_METHOD_HYBRID_PYRAMID_CONTRAST;
+    if (!hybrid) {
+        HIP_TRY(fg::launch_quantize(s->acc, s->W, s->H, s->sb, s->out_dev, s->stream), "quantize");
+    } else {
+        // src/stack/pipeline.zig:177-236: soft = the pyramid result, hard = winners, then merge
+        HIP_TRY(fg::launch_quantize(s->acc, s->W, s->H, s->sb, s->soft, s->stream), "quantize");
+        HIP_TRY(fg::launch_hybrid_merge(s->soft, s->hard, s->sb, &s->lt, 2, 4, s->opts.hybrid_sharpness, s->merge_gs,
+                                        s->merge_gh, s->acc, s->out_dev, s->stream),
+                "hybrid merge");
+    }
+    s->finished = true;
+    return FG_OK;
+}
+
+int fg_stack_finish_device(fg_stack* s, void* d_out) {
+    int rc = finish(s);
+    if (rc) return rc;
+    if (d_out)
+        HIP_TRY(hipMemcpyAsync(d_out, s->out_dev, (size_t)s->W * s->H * 3 * s->sb, hipMemcpyDeviceToDevice, s->stream),
+                "copy out");
+    return FG_OK;
+}
+
+int fg_stack_finish(fg_stack* s, void* out_px) {
+    if (!out_px) return fail(FG_ERR_INVALID_ARG, "null output");
+    int rc = finish(s);
+    if (rc) return rc;
+    HIP_TRY(hipMemcpyAsync(out_px, s->out_dev, (size_t)s->W * s->H * 3 * s->sb, hipMemcpyDeviceToHost, s->stream),
+            "copy out");
+    HIP_TRY(hipStreamSynchronize(s->stream), "finish");
+    return FG_OK;
+}
+
+int fg_stack_levels(const fg_stack* s, uint32_t* levels) {
+    if (!s || !levels) return fail(FG_ERR_INVALID_ARG, "null argument");
+    *levels = (uint32_t)s->lt.n;
+    return FG_OK;
+}
+
+int fg_stack_level_size(const fg_stack* s, uint32_t level, uint32_t* w, uint32_t* h) {
+    if (!s || !w || !h) return fail(FG_ERR_INVALID_ARG, "null argument");
+    if (level >= (uint32_t)s->lt.n) return fail(FG_ERR_INVALID_ARG, "level out of range");
+    *w = s->lt.lv[level].w;
+    *h = s->lt.lv[level].h;
+    return FG_OK;
+}
+
+void* fg_stack_stream(const fg_stack* s) { return s ? (void*)s->stream : nullptr; }
+
+int fg_stack_sync(fg_stack* s) {
+    if (!s) return fail(FG_ERR_INVALID_ARG, "null stack");
+    HIP_TRY(hipStreamSynchronize(s->stream), "sync");
+    return FG_OK;
+}
+
+int fg_stack_debug_readback(fg_stack* s, int which, uint32_t index, void* dst, size_t dst_bytes) {
+    if (!s || !dst) return fail(FG_ERR_INVALID_ARG, "null argument");
+    int rc = set_device(s->ctx->device);
+    if (rc) return rc;
+    const size_t px = (size_t)s->W * s->H;
+    auto copy = [&](const void* src, size_t bytes) -> int {
+        if (bytes > dst_bytes) return fail(FG_ERR_INVALID_ARG, "destination too small");
+        HIP_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, s->stream), "readback");
+        HIP_TRY(hipStreamSynchronize(s->stream), "readback");
+        return FG_OK;
+    };
+    switch (which) {
+        case FG_DEBUG_NORM_SUM:
+            return copy(s->S, px * 4);
+        case FG_DEBUG_UNION_SUPPORT: {
+            std::vector<uint8_t> u(px);
+            rc = FG_OK;
+            HIP_TRY(hipMemcpyAsync(u.data(), s->U, px, hipMemcpyDeviceToHost, s->stream), "readback");
+            HIP_TRY(hipStreamSynchronize(s->stream), "readback");
+            if (px * 4 > dst_bytes) return fail(FG_ERR_INVALID_ARG, "destination too small");
+            float* f = (float*)dst;
+            for (size_t i = 0; i < px; i++) f[i] = u[i] ? 1.0f : 0.0f;
+            return FG_OK;
+        }
+        case FG_DEBUG_WEIGHTS:
+            if (!s->cached || index >= s->added) return fail(FG_ERR_INVALID_ARG, "weights of that frame not held");
+            return copy(s->weights + index * px, px * 4);
+        case FG_DEBUG_ACC_LEVEL:
+        case FG_DEBUG_MASK_LEVEL:
+        case FG_DEBUG_GAUSS_LEVEL: {
+            if (index >= (uint32_t)s->lt.n) return fail(FG_ERR_INVALID_ARG, "level out of range");
+            const fg::Level& l = s->lt.lv[index];
+            size_t n = (size_t)l.w * l.h;
+            if (which == FG_DEBUG_ACC_LEVEL) {
+                const float* src = s->acc;
+                if (s->finished) {
+                    if (!s->accDbg) return fail(FG_ERR_STATE, "accumulator is collapsed in place by finish");
+                    src = s->accDbg;
+                }
+                float* tmp = nullptr;
+                HIP_TRY(hipMalloc(&tmp, n * 12), "hipMalloc");
+                hipError_t e = fg::launch_planar_to_interleaved(src + 3 * l.off, n, tmp, s->stream);
+                rc = e == hipSuccess ? copy(tmp, n * 12) : fail_hip(e, "readback");
+                (void)hipFree(tmp);
+                return rc;
+            }
+            if (s->opts.debug_frame < 0) return fail(FG_ERR_STATE, "stack was created without debug_frame");
+            if (which == FG_DEBUG_MASK_LEVEL) {
+                std::vector<float2> m(n);
+                HIP_TRY(hipMemcpyAsync(m.data(), s->dbgM + l.off, n * 8, hipMemcpyDeviceToHost, s->stream), "readback");
+                HIP_TRY(hipStreamSynchronize(s->stream), "readback");
+                if (n * 4 > dst_bytes) return fail(FG_ERR_INVALID_ARG, "destination too small");
+                for (size_t i = 0; i < n; i++) ((float*)dst)[i] = m[i].x;
+                return FG_OK;
+            }
+            std::vector<float4> g(n);
+            HIP_TRY(hipMemcpyAsync(g.data(), s->dbgG + l.off, n * 16, hipMemcpyDeviceToHost, s->stream), "readback");
+            HIP_TRY(hipStreamSynchronize(s->stream), "readback");
+            if (n * 12 > dst_bytes) return fail(FG_ERR_INVALID_ARG, "destination too small");
+            for (size_t i = 0; i < n; i++) {
+                ((float*)dst)[i * 3] = g[i].x;
+                ((float*)dst)[i * 3 + 1] = g[i].y;
+                ((float*)dst)[i * 3 + 2] = g[i].z;
+            }
+            return FG_OK;
+        }
+        case FG_DEBUG_SOFT:
+        case FG_DEBUG_HARD:
+            if (s->opts.method != FG_METHOD_HYBRID_PYRAMID_CONTRAST) return fail(FG_ERR_STATE, "not a hybrid stack");
+            return copy(which == FG_DEBUG_SOFT ? s->soft : s->hard, px * 3 * s->sb);
+        default:
+            return fail(FG_ERR_INVALID_ARG, "unknown debug buffer");
+    }
+}
+
+int fg_fuse_stack(fg_context* ctx, uint32_t n, const fg_image_desc* descs, const void* const* pxs,
+                  const fg_pose* poses, const fg_rect* roi, const fg_stack_opts* opts, void* out_px) {
+    if (!ctx || !descs || !pxs || !opts || !out_px || n == 0) return fail(FG_ERR_INVALID_ARG, "null argument");
+    fg_stack_opts o = *opts;
+    o.frame_count = n;
+    fg_stack* s = nullptr;
+    int rc = fg_stack_create(ctx, &o, &s);
+    if (rc) return rc;
+    for (uint32_t k = 0; k < n && rc == FG_OK; k++)
+        rc = fg_stack_add_frame(s, &descs[k], pxs[k], poses ? &poses[k] : nullptr, roi);
+    if (rc == FG_OK && fg_stack_needs_pass2_frames(s))
+        for (uint32_t k = 0; k < n && rc == FG_OK; k++)
+            rc = fg_stack_pass2_frame(s, &descs[k], pxs[k], poses ? &poses[k] : nullptr, roi);
+    if (rc == FG_OK) rc = fg_stack_finish(s, out_px);
+    fg_stack_destroy(s);
+    return rc;
+}
+
+// computeCommonOverlapRoi src/remap.zig:585-633 with transformPoint src/optimize.zig:3506-3525.
+static void transform_point(const fg_pose& p, double x, double y, uint32_t w, uint32_t h, double* ox, double* oy);
+
+int fg_common_overlap_roi(uint32_t n, const uint8_t* active, const uint32_t* widths, const uint32_t* heights,
+                          const fg_pose* poses, fg_rect* out) {
+    if (!widths || !heights || !poses || !out) return fail(FG_ERR_INVALID_ARG, "null argument");
+    struct V {
+        double x, y;
+    };
+    auto inside = [](V p, V a, V b) { return ((b.x - a.x) * (p.y - a.y) - (b.y - a.y) * (p.x - a.x)) >= -1e-6; };
+    auto isect = [](V p1, V p2, V q1, V q2) {
+        double a1 = p2.y - p1.y, b1 = p1.x - p2.x, c1 = a1 * p1.x + b1 * p1.y;
+        double a2 = q2.y - q1.y, b2 = q1.x - q2.x, c2 = a2 * q1.x + b2 * q1.y;
+        double det = a1 * b2 - a2 * b1;
+        if (fabs(det) < 1e-9) return p2;
+        return V{(b2 * c1 - b1 * c2) / det, (a1 * c2 - a2 * c1) / det};
+    };
+    std::vector<V> poly, next;
+    bool first = true;
+    for (uint32_t i = 0; i < n; i++) {
+        if (active && !active[i]) continue;
+        uint32_t w = widths[i], h = heights[i];
+        V q[4];
+        double xs[4] = {0, (double)(w - 1), (double)(w - 1), 0}, ys[4] = {0, 0, (double)(h - 1), (double)(h - 1)};
+        for (int k = 0; k < 4; k++) transform_point(poses[i], xs[k], ys[k], w, h, &q[k].x, &q[k].y);
+        if (first) {
+            poly.assign(q, q + 4);
+            first = false;
+            continue;
+        }
+        for (int e = 0; e < 4; e++) {
+            if (poly.empty()) break;
+            V a = q[e], b = q[(e + 1) % 4];
+            next.clear();
+            V prev = poly.back();
+            bool prev_in = inside(prev, a, b);
+            for (const V& cur : poly) {
+                bool cur_in = inside(cur, a, b);
+                if (cur_in != prev_in) next.push_back(isect(prev, cur, a, b));
+                if (cur_in) next.push_back(cur);
+                prev = cur;
+                prev_in = cur_in;
+            }
+            poly.swap(next);
+        }
+        if (poly.empty()) return 0;
+    }
+    if (poly.empty()) return 0;
+    double mnx = poly[0].x, mxx = poly[0].x, mny = poly[0].y, mxy = poly[0].y;
+    for (const V& p : poly) {
+        mnx = fmin(mnx, p.x);
+        mxx = fmax(mxx, p.x);
+        mny = fmin(mny, p.y);
+        mxy = fmax(mxy, p.y);
+    }
+    fg_rect r{(int32_t)ceil(mnx), (int32_t)ceil(mny), (int32_t)floor(mxx) + 1, (int32_t)floor(mxy) + 1};
+    if (r.right <= r.left || r.bottom <= r.top) return 0;
+    *out = r;
+    return 1;
+}
+
+}  // extern "C"
+
+// ---- forward transform (host) ----
+static void persp_sphere(double* x, double* y, const double m[3][3], double d) {  // perspSpherePoint :3272-3292
+    double px = *x, py = *y;
+    double r = sqrt(px * px + py * py);
+    double th = r / d;
+    double rs = r == 0.0 ? 0.0 : sin(th) / r;
+    double vx = rs * px, vy = rs * py, vz = cos(th);
+    double tx = m[0][0] * vx + m[1][0] * vy + m[2][0] * vz;
+    double ty = m[0][1] * vx + m[1][1] * vy + m[2][1] * vz;
+    double tz = m[0][2] * vx + m[1][2] * vy + m[2][2] * vz;
+    double r2 = sqrt(tx * tx + ty * ty);
+    double sc = r2 == 0.0 ? 0.0 : d * atan2(r2, tz) / r2;
+    *x = sc * tx;
+    *y = sc * ty;
+}
+
+static void transform_point(const fg_pose& p, double x, double y, uint32_t w, uint32_t h, double* ox, double* oy) {
+    double cx = ((double)w - 1.0) * 0.5, cy = ((double)h - 1.0) * 0.5;
+    double pano = focal_length_px(w, p.base_hfov_degrees);
+    double eff = p.base_hfov_degrees + p.hfov_delta;
+    double img_f = focal_length_px(w, eff < 1e-3 ? 1e-3 : (eff > 179.0 ? 179.0 : eff));
+    if (basic_rectilinear(p)) {  // basicRectilinearTransformPoint :3662-3683
+        double rs = pano / img_f;
+        double px = (x - cx) * rs, py = (y - cy) * rs;
+        double r = sqrt(px * px + py * py) / pano;  // sphereTpRectPoint :3238-3245
+        double th = r == 0.0 ? 1.0 : atan(r) / r;
+        px = th * px;
+        py = th * py;
+        double m[3][3];
+        set_matrix(p.pitch, 0.0, p.roll, m);
+        persp_sphere(&px, &py, m, pano);
+        double rr = sqrt(px * px + py * py);  // erectSphereTpPoint :3294-3307
+        double t = rr / pano;
+        double s = t == 0.0 ? 1.0 / pano : sin(t) / rr;
+        double v1 = s * px, v0 = cos(t);
+        double ey = pano * atan((s * py) / sqrt(v0 * v0 + v1 * v1));
+        double ex = pano * atan2(v1, v0);
+        double half = pano * kPi, full = 2.0 * half;  // rotateErectPoint :3347-3362
+        double xx = ex + p.yaw * pano;
+        if (xx < -half) {
+            xx += full;
+            while (xx < -half) xx += full;
+        } else if (xx > half) {
+            xx -= full;
+            while (xx > half) xx -= full;
+        }
+        double phi = xx / pano;  // rectErectPoint :3326-3335
+        if (phi < -kPi * 0.5 || phi > kPi * 0.5) {
+            *ox = -1.0;
+            *oy = -1.0;
+            return;
+        }
+        double theta = -(ey / pano) + kPi * 0.5;
+        *ox = cx + pano * tan(phi);
+        *oy = cy + pano / (tan(theta) * cos(phi));
+        return;
+    }
+    // sourceWorldRay :3781-3793 (applyRadialDistortion :4164-4172, rectilinearRay :3883-3889)
+    double dx = (x - cx) - p.center_shift_x, dy = (y - cy) - p.center_shift_y;
+    double hw = ((double)w - 1.0) * 0.5, hh = ((double)h - 1.0) * 0.5;
+    double dn = hw * hw + hh * hh;
+    double nr2 = (dx * dx + dy * dy) * (dn >= 1e-12 ? 1.0 / dn : 1.0);
+    double r4 = nr2 * nr2, r6 = r4 * nr2;
+    double factor = 1.0 + p.radial_a * nr2 + p.radial_b * r4 + p.radial_c * r6;
+    double v[3] = {dx * factor, dy * factor, -img_f};
+    double nrm = sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+    if (nrm < 1e-12) {
+        v[0] = 0;
+        v[1] = 0;
+        v[2] = 1;
+    } else {
+        v[0] = v[0] / nrm;
+        v[1] = v[1] / nrm;
+        v[2] = v[2] / nrm;
+    }
+    double m[3][3];
+    set_matrix(-p.pitch, 0.0, p.roll, m);
+    double t[3] = {m[0][0] * v[0] + m[1][0] * v[1] + m[2][0] * v[2], m[0][1] * v[0] + m[1][1] * v[1] + m[2][1] * v[2],
+                   m[0][2] * v[0] + m[1][2] * v[1] + m[2][2] * v[2]};
+    double c = cos(p.yaw), s = sin(p.yaw);
+    double ray[3] = {c * t[0] - s * t[2], t[1], s * t[0] + c * t[2]};
+    if (has_translation(p)) {  // projectViaTranslationPlane :3970-3978
+        double phi = p.translation_plane_yaw / 1.0;
+        double tz = kPi * 0.5 - (-p.translation_plane_pitch / 1.0);
+        double nv[3] = {sin(tz) * sin(phi), cos(tz), sin(tz) * -cos(phi)};
+        double cam[3] = {p.trans_x, p.trans_y, p.trans_z};
+        double tgt[3] = {cam[0] + ray[0], cam[1] + ray[1], cam[2] + ray[2]};
+        double dir[3] = {tgt[0] - cam[0], tgt[1] - cam[1], tgt[2] - cam[2]};
+        double num = 1.0 - (nv[0] * cam[0] + nv[1] * cam[1] + nv[2] * cam[2]);
+        double den = nv[0] * dir[0] + nv[1] * dir[1] + nv[2] * dir[2];
+        if (fabs(den) < 1e-12) {
+            ray[0] = 0.0;
+            ray[1] = 0.0;
+            ray[2] = 1.0;
+        } else {
+            double u = num / den;
+            for (int k = 0; k < 3; k++) ray[k] = cam[k] + u * dir[k];
+        }
+    }
+    double denom = -ray[2];  // projectRectilinear :4073-4080
+    if (denom <= 1e-12) {
+        *ox = -1.0;
+        *oy = -1.0;
+        return;
+    }
+    *ox = cx + pano * (ray[0] / denom);
+    *oy = cy + pano * (ray[1] / denom);
+}

@@ -1,0 +1,104 @@
+// Shared declarations of the HIP kernels (fg_kernels.hip) and the C-ABI host layer (fg_api.cpp).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+namespace fg {
+
+// How an output (panorama) pixel maps to a source pixel.
+enum RemapMode : int32_t {
+    REMAP_HOMOGRAPHY = 0,   // basic rectilinear (legacy chain == homography) or lens path without radial terms
+    REMAP_LENS = 1,         // lens path with radial terms: homography + Newton undistort
+    REMAP_TRANSLATION = 2,  // translation-plane path
+};
+
+// Per-frame constants of the inverse transform, prepared on the host from the pose.
+// local = B * (x_pano - dcx, y_pano - dcy, neg_d);  denom = -local.z
+// radial = focal * local.xy / denom + center_shift;   source = src_center + undistort(radial)
+struct RemapParams {
+    double B[9];
+    double neg_d;
+    double dcx, dcy;
+    double focal;
+    double csx, csy;
+    double scx, scy;
+    double ra, rb, rc, inv_norm;
+    double pn[3];
+    double tx, ty, tz;
+    int32_t mode;
+    int32_t legacy;   // basic rectilinear: only rays in front of the camera land in the image
+    int32_t radial;   // lens mode: radial terms present
+    int32_t roi_left, roi_top;
+};
+
+// Level l of a pyramid: size and the offset (in pixels) of its first pixel in a buffer that
+// stores all levels back to back.
+struct Level {
+    uint32_t w, h;
+    size_t off;
+};
+
+constexpr int kMaxLevels = 40;
+
+struct LevelTable {
+    int n;
+    Level lv[kMaxLevels];
+    size_t total;  // pixels over all levels
+};
+
+void make_levels(uint32_t w, uint32_t h, uint32_t max_levels, LevelTable* t);
+void make_remap_params(const double pose[15], uint32_t w, uint32_t h, int32_t roi_left, int32_t roi_top,
+                       RemapParams* p);
+
+// ---- launchers (all asynchronous on `s`) ----
+hipError_t launch_remap(const void* src, int sample_bytes, int channels, uint32_t sw, uint32_t sh,
+                        const RemapParams& p, uint32_t ow, uint32_t oh, void* dst, hipStream_t s);
+// RGB(A) aligned frame -> RGBA with alpha = max when the frame has no alpha
+hipError_t launch_to_rgba(const void* src, int sample_bytes, int extra, uint32_t w, uint32_t h, void* dst,
+                          hipStream_t s);
+
+struct Pass1Args {
+    const void* rgba;   // W*H*4 samples
+    int sample_bytes;
+    uint32_t w, h;
+    uint32_t window;
+    float* weights;     // out: contrast * alpha
+    float* norm_sum;    // S += w
+    uint8_t* union_sup; // U |= alpha > 0
+    float* best;        // hybrid: winner weight (nullptr if not hybrid)
+    void* hard;         // hybrid: winner RGB
+};
+hipError_t launch_pass1(const Pass1Args& a, hipStream_t s);
+
+struct Pass2Args {
+    const void* rgba;        // level-0 gaussian (RGB) and image support (alpha > 0)
+    int sample_bytes;
+    const float* weights;    // level-0 mask numerator
+    const float* norm_sum;
+    const uint8_t* union_sup;
+    uint32_t frame_count;
+    const LevelTable* lt;
+    float4* G;               // levels >= 1: rgb + support
+    float2* M;               // levels >= 1: mask + support
+    float* acc;              // planar accumulators: level l at 3*lv[l].off, planes R,G,B of lw*lh
+};
+hipError_t launch_pass2_frame(const Pass2Args& a, hipStream_t s);
+// debug: level-0 gaussian/mask of a frame into G[0]/M[0] slots of the debug buffers
+hipError_t launch_level0_debug(const Pass2Args& a, float4* G0, float2* M0, hipStream_t s);
+
+hipError_t launch_collapse(float* acc, const LevelTable* lt, hipStream_t s);
+hipError_t launch_quantize(const float* acc0, uint32_t w, uint32_t h, int sample_bytes, void* out, hipStream_t s);
+hipError_t launch_zero(void* p, size_t bytes, hipStream_t s);
+hipError_t launch_fill_f32(float* p, size_t n, float v, hipStream_t s);
+
+// hybrid merge (mergeHybridImages): Laplacian pyramids of soft and hard, mix, collapse, quantize
+hipError_t launch_hybrid_merge(const void* soft, const void* hard, int sample_bytes, const LevelTable* lt,
+                               uint32_t hard_start, uint32_t hard_count, float mix, float* gs, float* gh,
+                               float* out_pyr, void* out, hipStream_t s);
+
+// planar acc level -> interleaved host-order buffer (debug)
+hipError_t launch_planar_to_interleaved(const float* planar, size_t n, float* out, hipStream_t s);
+
+}  // namespace fg

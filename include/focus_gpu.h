@@ -1,0 +1,173 @@
+/*
+ * focus_gpu.h — C ABI of the MI355X (gfx950) remap + pyramid-contrast fusion library.
+ *
+ * Drop-in for the per-frame hot path of asingleoat/focus-stacker: the Zig callers
+ * keep their loops and swap the per-frame bodies for these calls, the same way they
+ * already bind C libraries with @cImport (src/fft_backend.zig:5-7, build.zig:266-282).
+ * Alignment (feature matching, phase correlation, the pose solve) stays on the CPU
+ * and hands in poses.
+ *
+ * Conventions
+ *   - C99, plain pointers and sizes.  Every call returns an int status: FG_OK (0) or a
+ *     negative FG_ERR_*; fg_last_error() gives a thread-local message for the last failure.
+ *   - Pixels are interleaved row-major, stride = width * channels samples, no padding,
+ *     exactly as image_io.Image (src/image_io.zig:31-60).
+ *   - Host-pointer calls copy their inputs before returning; `*_device` calls take
+ *     device pointers and enqueue on the stack's (or the given) HIP stream.
+ *   - One fg_stack is driven by one host thread at a time; a context may serve many.
+ */
+#ifndef FOCUS_GPU_H
+#define FOCUS_GPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum fg_status {
+    FG_OK = 0,
+    FG_ERR_INVALID_ARG = -1,
+    FG_ERR_OOM_HOST = -2,
+    FG_ERR_OOM_DEVICE = -3,
+    FG_ERR_SIZE_MISMATCH = -4,    /* error.MismatchedImageSizes  src/fuse/io.zig:6-9 */
+    FG_ERR_FORMAT_MISMATCH = -5,  /* error.MismatchedImageFormats src/fuse/io.zig:6-9 */
+    FG_ERR_UNSUPPORTED = -6,
+    FG_ERR_DEVICE = -7,
+    FG_ERR_STATE = -8             /* call out of order (e.g. finish before any frame) */
+};
+
+enum fg_sample_type { FG_SAMPLE_U8 = 0, FG_SAMPLE_U16 = 1 };   /* image_io.SampleType src/image_io.zig:25-28 */
+
+/* fuse.config.Method (src/fuse/config.zig:10-15); the pyramid methods are served here. */
+enum fg_method {
+    FG_METHOD_HARDMASK_CONTRAST = 0,
+    FG_METHOD_SOFTMASK_CONTRAST = 1,
+    FG_METHOD_PYRAMID_CONTRAST = 2,
+    FG_METHOD_HYBRID_PYRAMID_CONTRAST = 3
+};
+
+/* optimize.ImagePose: the 15 f64 fields in declaration order (src/optimize.zig:32-48). */
+typedef struct fg_pose {
+    double yaw, pitch, roll, hfov_delta;
+    double trans_x, trans_y, trans_z;
+    double translation_plane_yaw, translation_plane_pitch;
+    double radial_a, radial_b, radial_c;
+    double center_shift_x, center_shift_y;
+    double base_hfov_degrees;
+} fg_pose;
+
+/* remap.Rect (src/remap.zig:573-578): [left, right) x [top, bottom) in panorama pixels. */
+typedef struct fg_rect { int32_t left, top, right, bottom; } fg_rect;
+
+/* The parts of image_io.ImageInfo the path reads (src/image_io.zig:31-43). */
+typedef struct fg_image_desc {
+    uint32_t width, height;
+    uint32_t color_channels;   /* 1 (grayscale) or 3 (rgb); fusion needs 3 */
+    uint32_t extra_channels;   /* 0, or 1 = a support alpha follows the colour channels */
+    uint32_t sample_type;      /* enum fg_sample_type */
+} fg_image_desc;
+
+typedef struct fg_context fg_context;
+typedef struct fg_stack fg_stack;
+
+typedef struct fg_context_opts {
+    int32_t device;            /* HIP device ordinal */
+    float memory_fraction;     /* --memory-fraction (src/memory_budget.zig:3): share of free HBM a
+                                  stack may hold for its frame cache; 0 = no cap */
+} fg_context_opts;
+
+typedef struct fg_stack_opts {
+    uint32_t method;           /* enum fg_method: PYRAMID_CONTRAST or HYBRID_PYRAMID_CONTRAST */
+    uint32_t contrast_window_size; /* --contrast-window-size, odd, 3..31 (src/fuse/config.zig:295-299) */
+    float hybrid_sharpness;    /* --hybrid-sharpness in [0,1] (default 0.35, src/fuse/pyramid.zig:11) */
+    uint32_t frame_count;      /* N active frames (the 1/N fallback of normalizeWeightsInto) */
+    uint32_t width, height;    /* fused size = ROI size (or the aligned frame size) */
+    uint32_t sample_type;      /* enum fg_sample_type of the frames and of the output */
+    uint32_t max_levels;       /* 0 = full depth down to 1x1 (reference); else cap (enfuse --levels) */
+    float memory_fraction;     /* < 0: use the context's; else as fg_context_opts */
+    int32_t debug_frame;       /* >= 0: keep that frame's mask/gaussian levels for fg_stack_debug_readback */
+    void* stream;              /* hipStream_t to run on, or NULL for a stack-owned stream */
+} fg_stack_opts;
+
+enum fg_debug_buffer {
+    FG_DEBUG_NORM_SUM = 0,     /* f32 W*H   : sum of weights (src/stack/pipeline.zig:469) */
+    FG_DEBUG_UNION_SUPPORT = 1,/* f32 W*H   : 1 where any frame has alpha > 0 */
+    FG_DEBUG_WEIGHTS = 2,      /* f32 W*H   : frame `index` contrast * alpha (cached frames only) */
+    FG_DEBUG_ACC_LEVEL = 3,    /* f32 lw*lh*3 interleaved: accumulator level `index` (before finish) */
+    FG_DEBUG_MASK_LEVEL = 4,   /* f32 lw*lh : debug_frame's normalised mask level `index` */
+    FG_DEBUG_GAUSS_LEVEL = 5,  /* f32 lw*lh*3 interleaved: debug_frame's gaussian level `index` */
+    FG_DEBUG_SOFT = 6,         /* samples W*H*3: hybrid soft (pyramid) image (after finish) */
+    FG_DEBUG_HARD = 7          /* samples W*H*3: hybrid hard-mask image */
+};
+
+/* Message of the last failed call on this thread ("" if none). */
+const char* fg_last_error(void);
+/* Library version string. */
+const char* fg_version(void);
+
+int fg_context_create(const fg_context_opts* opts, fg_context** out);
+void fg_context_destroy(fg_context* ctx);
+
+/* remap.remapRigidImage (src/remap.zig:166-204): remap `src` under `pose` into `roi`
+ * (NULL = full frame).  dst receives roi_w*roi_h*(color_channels+1) samples, the last
+ * channel being the support alpha.  The source must have extra_channels == 0. */
+int fg_remap(fg_context* ctx, const fg_image_desc* src, const void* src_px, const fg_pose* pose,
+             const fg_rect* roi, void* dst_px);
+/* Same on device buffers, enqueued on `stream` (hipStream_t, NULL = the context's stream). */
+int fg_remap_device(fg_context* ctx, const fg_image_desc* src, const void* d_src, const fg_pose* pose,
+                    const fg_rect* roi, void* d_dst, void* stream);
+
+/* remap.computeCommonOverlapRoi (src/remap.zig:585-633), host arithmetic.  active may be
+ * NULL (all active).  Returns 1 and fills *out if the overlap is non-empty, 0 if not. */
+int fg_common_overlap_roi(uint32_t n, const uint8_t* active, const uint32_t* widths, const uint32_t* heights,
+                          const fg_pose* poses, fg_rect* out);
+
+/* Pyramid-contrast stack: the two-pass loop of runPyramidStackFusion
+ * (src/stack/pipeline.zig:361-605) / runPyramidPass (src/fuse/pipeline.zig:254-445). */
+int fg_stack_create(fg_context* ctx, const fg_stack_opts* opts, fg_stack** out);
+void fg_stack_destroy(fg_stack* stack);
+
+/* Pass 1 for one frame, in stack order.  With a pose the frame is a source image
+ * (extra_channels 0) remapped into `roi` (NULL = full frame); without a pose it is an
+ * already aligned image of the stack size (src/fuse/pipeline.zig:289-336). */
+int fg_stack_add_frame(fg_stack* stack, const fg_image_desc* desc, const void* px, const fg_pose* pose,
+                       const fg_rect* roi);
+int fg_stack_add_frame_device(fg_stack* stack, const fg_image_desc* desc, const void* d_px, const fg_pose* pose,
+                              const fg_rect* roi);
+
+/* 1 if the frame cache did not fit the memory budget: the caller must then feed every frame
+ * again, in the same order, through fg_stack_pass2_frame* before fg_stack_finish (the
+ * reference's uncached second loop, src/stack/pipeline.zig:561-597). */
+int fg_stack_needs_pass2_frames(const fg_stack* stack);
+int fg_stack_pass2_frame(fg_stack* stack, const fg_image_desc* desc, const void* px, const fg_pose* pose,
+                         const fg_rect* roi);
+int fg_stack_pass2_frame_device(fg_stack* stack, const fg_image_desc* desc, const void* d_px,
+                                const fg_pose* pose, const fg_rect* roi);
+
+/* Pass 2 (if cached), collapse, hybrid merge; writes W*H*3 samples (extra_channels = 0,
+ * src/fuse/pyramid.zig:252-253).  The _device form leaves the result in device memory and
+ * does not wait for it. */
+int fg_stack_finish(fg_stack* stack, void* out_px);
+int fg_stack_finish_device(fg_stack* stack, void* d_out);
+
+/* Debug readback (the --dump-masks-dir buffers, src/fuse/debug.zig). */
+int fg_stack_debug_readback(fg_stack* stack, int which, uint32_t index, void* host_dst, size_t dst_bytes);
+/* Number of pyramid levels of the stack and the size of level `level`. */
+int fg_stack_levels(const fg_stack* stack, uint32_t* levels);
+int fg_stack_level_size(const fg_stack* stack, uint32_t level, uint32_t* width, uint32_t* height);
+/* The HIP stream the stack runs on (hipStream_t). */
+void* fg_stack_stream(const fg_stack* stack);
+/* Blocks until the stack's queued work is done. */
+int fg_stack_sync(fg_stack* stack);
+
+/* One-shot: every frame through pass 1, pass 2 and collapse (poses may be NULL for
+ * aligned frames, roi may be NULL). */
+int fg_fuse_stack(fg_context* ctx, uint32_t n, const fg_image_desc* descs, const void* const* pxs,
+                  const fg_pose* poses, const fg_rect* roi, const fg_stack_opts* opts, void* out_px);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
