@@ -51,6 +51,18 @@ def _level_close(got, ref, rel=1e-5):
 # remap
 # --------------------------------------------------------------------------
 
+def _remap_close(got, ref):
+    """Alpha always compared; colour compared where either side has support.  A source
+    coordinate that lands exactly on the -1 boundary row/column (identity-like poses with
+    the ROI outside the frame) comes out of the reference's trig chain as -1 +- 1 ulp, so
+    there the colour of a zero-support sample is not defined by the reference either."""
+    unsupported = (got[..., -1] == 0) & (ref[..., -1] == 0)
+    g = got.astype(np.int64)
+    r = ref.astype(np.int64)
+    g[unsupported, :-1] = 0
+    r[unsupported, :-1] = 0
+    return _img_close(g, r, 1, 0.01)
+
 def test_golden_identity_remap(ctx, fg):
     """port3_0000.tif is the reference's own remap output of 0001.jpg under the identity pose."""
     d = np.load(os.path.join(GOLDEN, "s003_identity.npz"))
@@ -85,7 +97,7 @@ def test_remap_matches_oracle(ctx, fg, oracle, dtype, channels, kind):
     got = ctx.remap(src, fg.pose(POSES[kind]), roi)
     ref = oracle.remap(src, make_pose(**POSES[kind]), roi)
     assert got.shape == ref.shape
-    d = _img_close(got, ref, 1, 0.01)
+    d = _remap_close(got, ref)
     frac = float((d != 0).mean())
     print(f"remap {kind} ch={channels} {np.dtype(dtype).name}: differing samples {frac:.2e}")
     assert frac < 1e-3
@@ -98,7 +110,7 @@ def test_remap_real_solved_pose(ctx, fg, oracle):
     p = meta["poses"][1]
     got = ctx.remap(d["src"], fg.pose(p))
     ref = oracle.remap(d["src"], make_pose(**p))
-    dd = _img_close(got, ref, 1, 0.01)
+    dd = _remap_close(got, ref)
     assert (dd != 0).mean() < 1e-4
 
 
