@@ -1,0 +1,194 @@
+"""Benchmark: remap + pyramid-contrast fusion of one synthetic stack per step on one MI355X.
+
+Workload (N=1 line): 30 frames of 7952x5304 RGB u8 from the seeded synthetic generator
+(band-limited value-noise scene, depth ramp + Gaussian bump, per-frame defocus blur sigma up
+to 6 px, similarity poses with shifts U(+-20) px, roll U(+-0.3) deg, scale 1+U(+-0.01)),
+remapped into the common-overlap ROI and fused with the reference's full pyramid depth.
+Frames are generated on the GPU and stay resident in HBM; the timed region is K steps of
+{reset, 30 x add_frame_device (remap + pass 1), finish_device (pass 2 + collapse)}.
+
+Output: one JSON line (see DESIGN.md for the roofline byte model and the CPU baseline).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+for p in (os.path.join(ROOT, "focus-stacker_amd"), os.path.join(ROOT, "oracle")):
+    if p not in sys.path:
+        sys.path.insert(0, p)
+
+PEAK_HBM_GBS = 8000.0
+METRIC = "fused MPix/s (remap+pyramid-contrast), %HBM roofline; p50/p99 stack latency"
+
+
+def alg_bytes_per_frame_px(sample_bytes: int, n: int) -> float:
+    """Two-pass streaming model: pass 1 = 3s + 14, pass 2 = 3s + 41, collapse (12*4/3 + 3s)/N."""
+    s = sample_bytes
+    return (3 * s + 14) + (3 * s + 41) + (16 + 3 * s) / n
+
+
+def pct(v, q):
+    v = sorted(v)
+    if not v:
+        return None
+    k = min(len(v) - 1, max(0, int(round(q / 100.0 * (len(v) - 1)))))
+    return v[k]
+
+
+def cpu_baseline(frames_np, poses, roi, budget_s=30.0):
+    """The oracle (single-threaded C restatement) on a bounded sample of the same workload:
+    frames 0-1 at full size over the top half of the ROI."""
+    from oracle import Oracle, make_pose
+    o = Oracle()
+    l, t, r, b = roi
+    sub = (l, t, r, t + (b - t) // 2)
+    t0 = time.perf_counter()
+    o.stack(frames_np[:2], [make_pose(**p) for p in poses[:2]], sub)
+    dt = time.perf_counter() - t0
+    px = 2 * (sub[2] - sub[0]) * (sub[3] - sub[1])
+    return {"value": round(px / dt / 1e6, 3), "unit": "MPix/s", "cores": 1, "kind": "port",
+            "sample": f"oracle/focus_oracle.c, frames 0-1 of the workload at full size, ROI rows "
+                      f"{sub[1]}-{sub[3]} ({sub[2]-sub[0]}x{sub[3]-sub[1]}), remap+weights+pyramid+collapse, "
+                      f"{dt:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--frames", type=int, default=None, help="override the frame count (quick runs)")
+    ap.add_argument("--levels", type=int, default=0, help="0 = full depth (reference)")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--profile-only", action="store_true", help="one profiled step (for rocprofv3 runs)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build runs on one GPU; multi-GPU sharding is described in DESIGN.md "
+              "but not launched (--gpus must be 1)", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+    import focus_gpu as fg
+    import synth
+
+    dev = torch.device("cuda:0")
+    spec = synth.config_spec(1)
+    if args.frames:
+        spec.frames = args.frames
+    n = spec.frames
+    print(f"bench: generating {n} x {spec.width}x{spec.height} u8 frames on the GPU", file=sys.stderr, flush=True)
+    gen = synth.Stack(spec, device=dev)
+    frames = [gen.frame(k) for k in range(n)]
+    poses = gen.poses
+    del gen
+    torch.cuda.synchronize()
+    torch.cuda.empty_cache()
+    fposes = [fg.pose(p) for p in poses]
+    roi = fg.common_overlap_roi([(spec.width, spec.height)] * n, fposes)
+    w, h = roi[2] - roi[0], roi[3] - roi[1]
+    desc = fg.ImageDesc(spec.width, spec.height, 3, 0, fg.SAMPLE_U8)
+    ctx = fg.Context(0, memory_fraction=0.8)
+    stack = fg.Stack(ctx, n, w, h, fg.SAMPLE_U8, max_levels=args.levels)
+    levels = len(stack.levels())
+    out = torch.empty((h, w, 3), dtype=torch.uint8, device=dev)
+    ext = torch.cuda.ExternalStream(stack.stream())
+
+    def step():
+        stack.reset()
+        for k in range(n):
+            stack.add_frame_device(frames[k].data_ptr(), desc, fposes[k], roi)
+        stack.finish_device(out.data_ptr())
+
+    if args.profile_only:
+        step()
+        stack.sync()
+        print(json.dumps({"profile_step": "done", "roi": roi, "levels": levels}), flush=True)
+        return
+
+    print(f"bench: roi={roi} ({w}x{h}), {levels} levels, warmup {args.warmup}", file=sys.stderr, flush=True)
+    for _ in range(args.warmup):
+        step()
+    stack.sync()
+    torch.cuda.synchronize()
+
+    lat = []
+    evs = []
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        a = torch.cuda.Event(enable_timing=True)
+        b = torch.cuda.Event(enable_timing=True)
+        a.record(ext)
+        step()
+        b.record(ext)
+        evs.append((a, b))
+    stack.sync()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    lat = [a.elapsed_time(b) for a, b in evs]
+    ms_per_step = (t1 - t0) * 1000.0 / args.steps
+    frame_px = n * w * h
+    value = frame_px / (ms_per_step / 1000.0) / 1e6
+
+    # one profiled step: per-phase device time from HIP events on the stack's stream
+    stack.set_profiling(True)
+    step()
+    ph = stack.phase_times()
+    stack.set_profiling(False)
+    print(f"bench: {ms_per_step:.2f} ms/step, {value:.1f} MPix/s, phases {ph}", file=sys.stderr, flush=True)
+
+    # roofline of the dominant phase with the per-unit algorithmic bytes of the terms it carries
+    s_b = 1
+    phase_bytes = {  # bytes per frame-pixel (per stack-pixel for finish), see DESIGN.md
+        "ingest": 3 * s_b + 4 * s_b,          # remap: read RGB source, write the RGBA frame
+        "pass1": 4 * s_b + 4 + 8 + 2,        # read RGBA, write w, RMW S, RMW U
+        "pass2": 3 * s_b + 41,               # pass-2 term of the streaming model
+        "finish": 16 + 3 * s_b,              # collapse: read accumulators, write the image
+    }
+    dom = max(("ingest", "pass1", "pass2"), key=lambda k: ph[k][0])
+    launches = max(ph[dom][1], 1)
+    avg_ms = ph[dom][0] / launches
+    units = w * h
+    ach = phase_bytes[dom] * units / (avg_ms / 1000.0) / 1e9
+    pipe_bytes = alg_bytes_per_frame_px(s_b, n) * frame_px
+    pipe_gbs = pipe_bytes / (ms_per_step / 1000.0) / 1e9
+
+    line = {
+        "metric": METRIC,
+        "value": round(value, 2),
+        "unit": "MPix/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u8",
+        "data": "synthetic",
+        "config": {"workload": f"{n} x 7952x5304 RGB u8 stack (synthetic, seeded), remap + pyramid-contrast, "
+                               f"{levels} levels", "frames": n, "source": [spec.width, spec.height],
+                   "roi": list(roi), "levels": levels, "parallelism": "single GPU"},
+        "latency_ms": {"p50": round(pct(lat, 50), 3), "p99": round(pct(lat, 99), 3)},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(ach, 1), "peak": PEAK_HBM_GBS,
+                     "unit": "GB/s", "frac": round(ach / PEAK_HBM_GBS, 4), "traffic": None,
+                     "avg_launch_ms": round(avg_ms, 4)},
+        "pipeline": {"alg_bytes_per_frame_px": round(alg_bytes_per_frame_px(s_b, n), 3),
+                     "achieved_GBs": round(pipe_gbs, 1), "frac": round(pipe_gbs / PEAK_HBM_GBS, 4)},
+        "phases_ms": {k: round(v[0], 3) for k, v in ph.items()},
+    }
+    if not args.no_cpu_baseline:
+        del frames[2:]
+        torch.cuda.synchronize()
+        print("bench: timing the CPU baseline (oracle) on a bounded sample", file=sys.stderr, flush=True)
+        frames_np = [f.cpu().numpy() for f in frames[:2]]
+        line["cpu_baseline"] = cpu_baseline(frames_np, poses, roi)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

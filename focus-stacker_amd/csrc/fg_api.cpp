@@ -162,6 +162,9 @@ struct fg_context {
     hipStream_t stream;
 };
 
+// One stack: the pass-1 / pass-2 driver.  Pass 1 runs over groups of frames (all pending frames
+// at finish, or whenever the host staging slots are full); pass 2 runs over every cached frame
+// at finish, or frame by frame when the frames are fed a second time.
 struct fg_stack {
     fg_context* ctx;
     fg_stack_opts opts;
@@ -171,31 +174,81 @@ struct fg_stack {
     int sb;
     fg::LevelTable lt;
     bool cached;
-    uint32_t added, pass2_done;
+    uint32_t added, p1_done, pass2_done;
     bool finished;
+    uint32_t src_w, src_h, src_cc, src_extra;  // geometry of the first frame (later frames must match)
+    bool have_src;
+    std::vector<fg::FrameDesc> fdesc;
     // device memory
     std::vector<void*> allocs;
-    uint8_t* frames;      // cached RGBA frames (or one slot)
-    float* weights;       // cached weights (or one slot)
+    fg::FrameDesc* d_fdesc;
+    uint8_t* frames;      // RGBA cache slots
+    float* weights;       // weight cache slots
+    float4* lev;          // per-frame levels >= 1
+    size_t lev_stride;
+    uint32_t slots;
     float* S;
     uint8_t* U;
     float* acc;
-    float4* G;
-    float2* M;
+    float* contrib;
+    size_t contrib_floats;
     float* best;
     uint8_t* hard;
     uint8_t* soft;
-    uint8_t* src_stage;
-    size_t src_stage_bytes;
-    float4* dbgG;
-    float2* dbgM;
+    std::vector<uint8_t*> stage;  // device copies of host-fed source frames awaiting pass 1
+    size_t stage_bytes;
+    uint32_t stage_used;
     float* accDbg;
     uint8_t* out_dev;
     float* merge_gs;
     float* merge_gh;
+    // profiling: event pairs per phase, resolved on query
+    bool profiling;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> ev[FG_PHASE_COUNT];
+    std::vector<hipEvent_t> ev_pool;
 };
 
 namespace {
+
+constexpr uint32_t kStageSlots = 8;
+
+hipEvent_t take_event(fg_stack* s) {
+    hipEvent_t e = nullptr;
+    if (!s->ev_pool.empty()) {
+        e = s->ev_pool.back();
+        s->ev_pool.pop_back();
+        return e;
+    }
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+}
+
+// Brackets the work enqueued in its scope with two events on the stack's stream.
+struct PhaseMark {
+    fg_stack* s;
+    int phase;
+    hipEvent_t a;
+    PhaseMark(fg_stack* st, int ph) : s(st), phase(ph), a(nullptr) {
+        if (s->profiling && (a = take_event(s))) (void)hipEventRecord(a, s->stream);
+    }
+    ~PhaseMark() {
+        if (!a) return;
+        hipEvent_t b = take_event(s);
+        if (!b) return;
+        (void)hipEventRecord(b, s->stream);
+        s->ev[phase].emplace_back(a, b);
+    }
+};
+
+void recycle_events(fg_stack* s) {
+    for (int k = 0; k < FG_PHASE_COUNT; k++) {
+        for (auto& pr : s->ev[k]) {
+            s->ev_pool.push_back(pr.first);
+            s->ev_pool.push_back(pr.second);
+        }
+        s->ev[k].clear();
+    }
+}
 
 size_t frame_bytes(const fg_stack* s) { return (size_t)s->W * s->H * 4 * s->sb; }
 size_t weight_bytes(const fg_stack* s) { return (size_t)s->W * s->H * sizeof(float); }
@@ -215,6 +268,8 @@ int dev_alloc(fg_stack* s, size_t bytes, void** out) {
 void free_stack(fg_stack* s) {
     if (!s) return;
     if (s->stream) (void)hipStreamSynchronize(s->stream);
+    recycle_events(s);
+    for (hipEvent_t e : s->ev_pool) (void)hipEventDestroy(e);
     for (void* p : s->allocs) (void)hipFree(p);
     if (s->own_stream && s->stream) (void)hipStreamDestroy(s->stream);
     delete s;
@@ -241,66 +296,125 @@ int set_device(int dev) {
     return FG_OK;
 }
 
-// Brings one frame (source + pose, or an aligned frame) into the RGBA slot `dst` on the stack's stream.
-int ingest_frame(fg_stack* s, const fg_image_desc* d, const void* px, bool device_ptr, const fg_pose* pose,
-                 const fg_rect* roi, uint8_t* dst) {
+// Validates frame k and fills its pass-1 descriptor; host pixels are copied to a staging slot.
+int describe_frame(fg_stack* s, const fg_image_desc* d, const void* px, bool device_ptr, const fg_pose* pose,
+                   const fg_rect* roi, fg::FrameDesc* f) {
     int rc = check_desc(d);
     if (rc) return rc;
     if (!px) return fail(FG_ERR_INVALID_ARG, "null pixel pointer");
     int sb = d->sample_type == FG_SAMPLE_U16 ? 2 : 1;
     if (sb != s->sb) return fail(FG_ERR_FORMAT_MISMATCH, "frame sample type differs from the stack's");
     if (d->color_channels != 3) return fail(FG_ERR_UNSUPPORTED, "pyramid fusion needs rgb frames");
-    const void* dsrc = px;
-    if (!device_ptr) {
-        size_t nb = desc_bytes(d);
-        if (nb > s->src_stage_bytes) {
-            void* p = nullptr;
-            rc = dev_alloc(s, nb, &p);
-            if (rc) return rc;
-            s->src_stage = (uint8_t*)p;
-            s->src_stage_bytes = nb;
-        }
-        HIP_TRY(hipMemcpyAsync(s->src_stage, px, nb, hipMemcpyHostToDevice, s->stream), "upload frame");
-        HIP_TRY(hipStreamSynchronize(s->stream), "upload frame");
-        dsrc = s->src_stage;
+    // every frame of a stack has the first frame's size and format (src/fuse/io.zig:31-42)
+    if (s->have_src) {
+        if (d->width != s->src_w || d->height != s->src_h)
+            return fail(FG_ERR_SIZE_MISMATCH, "frame size differs from the first frame's");
+        if (d->extra_channels != s->src_extra) return fail(FG_ERR_FORMAT_MISMATCH, "frame channels differ");
     }
+    memset(f, 0, sizeof *f);
+    f->sw = d->width;
+    f->sh = d->height;
     if (pose) {
         if (d->extra_channels != 0) return fail(FG_ERR_UNSUPPORTED, "remap sources carry no alpha channel");
         fg_rect r = roi ? *roi : fg_rect{0, 0, (int32_t)d->width, (int32_t)d->height};
-        if ((uint32_t)(r.right - r.left) != s->W || (uint32_t)(r.bottom - r.top) != s->H || r.right <= r.left ||
-            r.bottom <= r.top)
+        if (r.right <= r.left || r.bottom <= r.top || (uint32_t)(r.right - r.left) != s->W ||
+            (uint32_t)(r.bottom - r.top) != s->H)
             return fail(FG_ERR_SIZE_MISMATCH, "roi size differs from the stack size");
-        fg::RemapParams p;
-        fg::make_remap_params((const double*)pose, d->width, d->height, r.left, r.top, &p);
-        HIP_TRY(fg::launch_remap(dsrc, sb, 3, d->width, d->height, p, s->W, s->H, dst, s->stream), "remap kernel");
+        fg::make_remap_params((const double*)pose, d->width, d->height, r.left, r.top, &f->p);
+        f->kind = 0;
     } else {
         if (d->width != s->W || d->height != s->H)
             return fail(FG_ERR_SIZE_MISMATCH, "aligned frame size differs from the stack size");
-        HIP_TRY(fg::launch_to_rgba(dsrc, sb, (int)d->extra_channels, s->W, s->H, dst, s->stream), "to_rgba kernel");
+        f->kind = 1;
+        f->extra = (int32_t)d->extra_channels;
+    }
+    if (device_ptr) {
+        f->src = px;
+    } else {
+        size_t nb = desc_bytes(d);
+        if (s->stage.empty() || nb > s->stage_bytes) {
+            if (!s->stage.empty()) return fail(FG_ERR_SIZE_MISMATCH, "frame larger than the first frame");
+            for (uint32_t j = 0; j < kStageSlots; j++) {
+                void* p = nullptr;
+                rc = dev_alloc(s, nb, &p);
+                if (rc) return rc;
+                s->stage.push_back((uint8_t*)p);
+            }
+            s->stage_bytes = nb;
+        }
+        uint8_t* slot = s->stage[s->stage_used];
+        HIP_TRY(hipMemcpyAsync(slot, px, nb, hipMemcpyHostToDevice, s->stream), "upload frame");
+        f->src = slot;
+    }
+    if (!s->have_src) {
+        s->have_src = true;
+        s->src_w = d->width;
+        s->src_h = d->height;
+        s->src_cc = d->color_channels;
+        s->src_extra = d->extra_channels;
     }
     return FG_OK;
 }
 
-int run_pass2(fg_stack* s, uint32_t k, const uint8_t* rgba, const float* w) {
-    fg::Pass2Args a{rgba, s->sb, w, s->S, s->U, s->opts.frame_count, &s->lt, s->G, s->M, s->acc};
-    HIP_TRY(fg::launch_pass2_frame(a, s->stream), "pass2 kernels");
-    if (s->opts.debug_frame >= 0 && (uint32_t)s->opts.debug_frame == k) {
-        HIP_TRY(fg::launch_level0_debug(a, s->dbgG, s->dbgM, s->stream), "debug level0");
-        size_t rest = s->lt.total - s->lt.lv[0].w * (size_t)s->lt.lv[0].h;
-        size_t off = s->lt.lv[0].w * (size_t)s->lt.lv[0].h;
-        if (rest) {
-            HIP_TRY(hipMemcpyAsync(s->dbgG + off, s->G + off, rest * sizeof(float4), hipMemcpyDeviceToDevice,
-                                   s->stream),
-                    "debug copy");
-            HIP_TRY(hipMemcpyAsync(s->dbgM + off, s->M + off, rest * sizeof(float2), hipMemcpyDeviceToDevice,
-                                   s->stream),
-                    "debug copy");
-        }
+// Runs pass 1 over the frames added since the last flush.
+int flush_pass1(fg_stack* s) {
+    if (s->p1_done >= s->added) return FG_OK;
+    const uint32_t k0 = s->p1_done, k1 = s->added;
+    HIP_TRY(hipMemcpyAsync(s->d_fdesc + k0, s->fdesc.data() + k0, (k1 - k0) * sizeof(fg::FrameDesc),
+                           hipMemcpyHostToDevice, s->stream),
+            "frame descriptors");
+    fg::Pass1Args a{};
+    a.frames = s->d_fdesc;
+    a.k0 = (int)k0;
+    a.k1 = (int)k1;
+    a.slot_base = s->cached ? 0 : (int)k0;
+    a.sample_bytes = s->sb;
+    a.w = s->W;
+    a.h = s->H;
+    a.window = s->opts.contrast_window_size;
+    a.rgba = s->frames;
+    a.weights = s->weights;
+    a.norm_sum = s->S;
+    a.union_sup = s->U;
+    a.best = s->best;
+    a.hard = s->hard;
+    {
+        PhaseMark pm(s, FG_PHASE_PASS1);
+        HIP_TRY(fg::launch_pass1(a, s->stream), "pass1 kernel");
     }
+    s->p1_done = k1;
+    s->stage_used = 0;
+    return FG_OK;
+}
+
+fg::Pass2Args pass2_args(fg_stack* s, int k0, int k1, int slot_base) {
+    fg::Pass2Args a{};
+    a.sample_bytes = s->sb;
+    a.lt = &s->lt;
+    a.k0 = k0;
+    a.k1 = k1;
+    a.slot_base = slot_base;
+    a.rgba = s->frames;
+    a.weights = s->weights;
+    a.norm_sum = s->S;
+    a.union_sup = s->U;
+    a.frame_count = s->opts.frame_count;
+    a.lev = s->lev;
+    a.lev_stride = s->lev_stride;
+    a.acc = s->acc;
+    a.contrib = s->contrib;
+    a.contrib_floats = s->contrib_floats;
+    return a;
+}
+
+int run_pass2(fg_stack* s, int k0, int k1, int slot_base) {
+    PhaseMark pm(s, FG_PHASE_PASS2);
+    HIP_TRY(fg::launch_pass2(pass2_args(s, k0, k1, slot_base), s->stream), "pass2 kernels");
     return FG_OK;
 }
 
 }  // namespace
+
 
 extern "C" {
 
@@ -415,6 +529,7 @@ int fg_stack_create(fg_context* ctx, const fg_stack_opts* o, fg_stack** out) {
     s->H = o->height;
     s->sb = o->sample_type == FG_SAMPLE_U16 ? 2 : 1;
     fg::make_levels(s->W, s->H, o->max_levels, &s->lt);
+    s->fdesc.resize(o->frame_count);
     if (o->stream) {
         s->stream = (hipStream_t)o->stream;
         s->own_stream = false;
@@ -429,16 +544,18 @@ int fg_stack_create(fg_context* ctx, const fg_stack_opts* o, fg_stack** out) {
     const bool hybrid = o->method == FG_METHOD_HYBRID_PYRAMID_CONTRAST;
     const size_t px = (size_t)s->W * s->H;
     const size_t tot = s->lt.total;
-    size_t fixed = px * 4 + px + tot * 12 + tot * 16 + tot * 8 + px * 3 * s->sb;
+    s->lev_stride = tot - px;  // levels >= 1
+    s->contrib_floats = fg::pass2_contrib_floats(&s->lt, o->frame_count);
+    size_t fixed = px * 4 + px + tot * 12 + s->contrib_floats * 4 + px * 3 * s->sb + o->frame_count * sizeof(fg::FrameDesc);
     if (hybrid) fixed += px * 4 + 2 * px * 3 * s->sb + 2 * tot * 12;
-    if (o->debug_frame >= 0) fixed += tot * 36;
-    size_t per_frame = frame_bytes(s) + weight_bytes(s);
+    if (o->debug_frame >= 0) fixed += tot * 12;
+    const size_t per_frame = frame_bytes(s) + weight_bytes(s) + s->lev_stride * sizeof(float4);
     size_t free_b = 0, total_b = 0;
     (void)hipMemGetInfo(&free_b, &total_b);
     float frac = o->memory_fraction >= 0.0f ? o->memory_fraction : ctx->memory_fraction;
     double budget = frac > 0.0f ? (double)frac * (double)free_b : (double)free_b - 1.0e9;
     s->cached = (double)fixed + (double)o->frame_count * (double)per_frame <= budget;
-    size_t slots = s->cached ? o->frame_count : 1;
+    s->slots = s->cached ? o->frame_count : 1;
     void* p;
 #define ALLOC(field, bytes)                      \
     do {                                         \
@@ -449,13 +566,14 @@ int fg_stack_create(fg_context* ctx, const fg_stack_opts* o, fg_stack** out) {
         }                                        \
         s->field = (decltype(s->field))p;        \
     } while (0)
-    ALLOC(frames, slots * frame_bytes(s));
-    ALLOC(weights, slots * weight_bytes(s));
+    ALLOC(d_fdesc, o->frame_count * sizeof(fg::FrameDesc));
+    ALLOC(frames, s->slots * frame_bytes(s));
+    ALLOC(weights, s->slots * weight_bytes(s));
+    ALLOC(lev, s->slots * s->lev_stride * sizeof(float4));
     ALLOC(S, px * sizeof(float));
     ALLOC(U, px);
     ALLOC(acc, tot * 3 * sizeof(float));
-    ALLOC(G, tot * sizeof(float4));
-    ALLOC(M, tot * sizeof(float2));
+    ALLOC(contrib, s->contrib_floats * sizeof(float));
     ALLOC(out_dev, px * 3 * s->sb);
     if (hybrid) {
         ALLOC(best, px * sizeof(float));
@@ -464,18 +582,10 @@ int fg_stack_create(fg_context* ctx, const fg_stack_opts* o, fg_stack** out) {
         ALLOC(merge_gs, tot * 3 * sizeof(float));
         ALLOC(merge_gh, tot * 3 * sizeof(float));
     }
-    if (o->debug_frame >= 0) {
-        ALLOC(dbgG, tot * sizeof(float4));
-        ALLOC(dbgM, tot * sizeof(float2));
-        ALLOC(accDbg, tot * 3 * sizeof(float));
-    }
+    if (o->debug_frame >= 0) ALLOC(accDbg, tot * 3 * sizeof(float));
 #undef ALLOC
-    hipError_t e = fg::launch_zero(s->S, px * sizeof(float), s->stream);
-    if (e == hipSuccess) e = fg::launch_zero(s->U, px, s->stream);
-    if (e == hipSuccess) e = fg::launch_zero(s->acc, tot * 3 * sizeof(float), s->stream);
-    if (e == hipSuccess && hybrid) e = fg::launch_fill_f32(s->best, px, -INFINITY, s->stream);
-    if (e != hipSuccess) {
-        rc = fail_hip(e, "stack init");
+    rc = fg_stack_reset(s);
+    if (rc) {
         free_stack(s);
         return rc;
     }
@@ -497,13 +607,15 @@ static int add_frame(fg_stack* s, const fg_image_desc* d, const void* px, bool d
     int rc = set_device(s->ctx->device);
     if (rc) return rc;
     const uint32_t k = s->added;
-    uint8_t* slot = s->frames + (s->cached ? k : 0) * frame_bytes(s);
-    float* w = s->weights + (s->cached ? k : 0) * (size_t)s->W * s->H;
-    rc = ingest_frame(s, d, px, dev, pose, roi, slot);
+    {
+        PhaseMark pm(s, FG_PHASE_INGEST);
+        rc = describe_frame(s, d, px, dev, pose, roi, &s->fdesc[k]);
+    }
     if (rc) return rc;
-    fg::Pass1Args a{slot, s->sb, s->W, s->H, s->opts.contrast_window_size, w, s->S, s->U, s->best, s->hard};
-    HIP_TRY(fg::launch_pass1(a, s->stream), "pass1 kernel");
+    if (!dev) s->stage_used++;
     s->added++;
+    // uncached stacks keep one frame slot: run pass 1 now; otherwise when the staging is full
+    if (!s->cached || s->stage_used == kStageSlots) return flush_pass1(s);
     return FG_OK;
 }
 
@@ -526,14 +638,33 @@ static int pass2_frame(fg_stack* s, const fg_image_desc* d, const void* px, bool
     if (s->pass2_done >= s->opts.frame_count) return fail(FG_ERR_STATE, "more pass-2 frames than frame_count");
     int rc = set_device(s->ctx->device);
     if (rc) return rc;
-    rc = ingest_frame(s, d, px, dev, pose, roi, s->frames);
+    const uint32_t k = s->pass2_done;
+    {
+        PhaseMark pm(s, FG_PHASE_INGEST);
+        rc = describe_frame(s, d, px, dev, pose, roi, &s->fdesc[k]);
+    }
     if (rc) return rc;
-    // The weight map is recomputed as in pass 1 (computeWeightMapForRemapped + applySupport,
-    // src/stack/pipeline.zig:571-572); S and U are final, so they are left untouched.
-    fg::Pass1Args a{s->frames, s->sb, s->W, s->H, s->opts.contrast_window_size, s->weights, nullptr, nullptr,
-                    nullptr, nullptr};
-    HIP_TRY(fg::launch_pass1(a, s->stream), "pass1 kernel");
-    rc = run_pass2(s, s->pass2_done, s->frames, s->weights);
+    // the frame's RGBA and weight map are rebuilt as in pass 1 (src/stack/pipeline.zig:566-572),
+    // leaving the final norm sum and union support untouched
+    HIP_TRY(hipMemcpyAsync(s->d_fdesc + k, &s->fdesc[k], sizeof(fg::FrameDesc), hipMemcpyHostToDevice, s->stream),
+            "frame descriptor");
+    fg::Pass1Args a{};
+    a.frames = s->d_fdesc;
+    a.k0 = (int)k;
+    a.k1 = (int)k + 1;
+    a.slot_base = (int)k;
+    a.sample_bytes = s->sb;
+    a.w = s->W;
+    a.h = s->H;
+    a.window = s->opts.contrast_window_size;
+    a.rgba = s->frames;
+    a.weights = s->weights;
+    {
+        PhaseMark pm(s, FG_PHASE_PASS1);
+        HIP_TRY(fg::launch_pass1(a, s->stream), "pass1 kernel");
+    }
+    s->stage_used = 0;
+    rc = run_pass2(s, (int)k, (int)k + 1, (int)k);
     if (rc) return rc;
     s->pass2_done++;
     return FG_OK;
@@ -553,14 +684,15 @@ static int finish(fg_stack* s) {
     if (s->added != s->opts.frame_count) return fail(FG_ERR_STATE, "not every frame went through pass 1");
     int rc = set_device(s->ctx->device);
     if (rc) return rc;
+    rc = flush_pass1(s);
+    if (rc) return rc;
     if (s->cached) {
-        for (uint32_t k = 0; k < s->opts.frame_count; k++) {
-            rc = run_pass2(s, k, s->frames + k * frame_bytes(s), s->weights + (size_t)k * s->W * s->H);
-            if (rc) return rc;
-        }
+        rc = run_pass2(s, 0, (int)s->opts.frame_count, 0);
+        if (rc) return rc;
     } else if (s->pass2_done != s->opts.frame_count) {
         return fail(FG_ERR_STATE, "uncached stack: feed every frame through fg_stack_pass2_frame first");
     }
+    PhaseMark pm(s, FG_PHASE_FINISH);
     if (s->accDbg)
         HIP_TRY(hipMemcpyAsync(s->accDbg, s->acc, s->lt.total * 3 * sizeof(float), hipMemcpyDeviceToDevice, s->stream),
                 "debug copy");
@@ -631,12 +763,14 @@ int fg_stack_debug_readback(fg_stack* s, int which, uint32_t index, void* dst, s
         HIP_TRY(hipStreamSynchronize(s->stream), "readback");
         return FG_OK;
     };
+    // pass 1 of pending frames has to have run for the weights / sums to be meaningful
+    rc = flush_pass1(s);
+    if (rc) return rc;
     switch (which) {
         case FG_DEBUG_NORM_SUM:
             return copy(s->S, px * 4);
         case FG_DEBUG_UNION_SUPPORT: {
             std::vector<uint8_t> u(px);
-            rc = FG_OK;
             HIP_TRY(hipMemcpyAsync(u.data(), s->U, px, hipMemcpyDeviceToHost, s->stream), "readback");
             HIP_TRY(hipStreamSynchronize(s->stream), "readback");
             if (px * 4 > dst_bytes) return fail(FG_ERR_INVALID_ARG, "destination too small");
@@ -652,7 +786,7 @@ int fg_stack_debug_readback(fg_stack* s, int which, uint32_t index, void* dst, s
         case FG_DEBUG_GAUSS_LEVEL: {
             if (index >= (uint32_t)s->lt.n) return fail(FG_ERR_INVALID_ARG, "level out of range");
             const fg::Level& l = s->lt.lv[index];
-            size_t n = (size_t)l.w * l.h;
+            const size_t n = (size_t)l.w * l.h;
             if (which == FG_DEBUG_ACC_LEVEL) {
                 const float* src = s->acc;
                 if (s->finished) {
@@ -666,23 +800,36 @@ int fg_stack_debug_readback(fg_stack* s, int which, uint32_t index, void* dst, s
                 (void)hipFree(tmp);
                 return rc;
             }
-            if (s->opts.debug_frame < 0) return fail(FG_ERR_STATE, "stack was created without debug_frame");
-            if (which == FG_DEBUG_MASK_LEVEL) {
-                std::vector<float2> m(n);
-                HIP_TRY(hipMemcpyAsync(m.data(), s->dbgM + l.off, n * 8, hipMemcpyDeviceToHost, s->stream), "readback");
+            const int f = s->opts.debug_frame;
+            if (f < 0) return fail(FG_ERR_STATE, "stack was created without debug_frame");
+            if (!s->cached) return fail(FG_ERR_STATE, "frame levels are kept for cached stacks only");
+            if (!s->finished) return fail(FG_ERR_STATE, "frame levels exist after fg_stack_finish");
+            std::vector<float4> v(n);
+            if (index == 0) {
+                float4* tmp = nullptr;
+                HIP_TRY(hipMalloc(&tmp, n * sizeof(float4)), "hipMalloc");
+                fg::Pass2Args a = pass2_args(s, 0, (int)s->opts.frame_count, 0);
+                hipError_t e = fg::launch_level0_encode(a, f, tmp, s->stream);
+                if (e == hipSuccess) e = hipMemcpyAsync(v.data(), tmp, n * sizeof(float4), hipMemcpyDeviceToHost, s->stream);
+                if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+                (void)hipFree(tmp);
+                if (e != hipSuccess) return fail_hip(e, "readback");
+            } else {
+                const float4* src = s->lev + (size_t)f * s->lev_stride + (l.off - s->lt.lv[1].off);
+                HIP_TRY(hipMemcpyAsync(v.data(), src, n * sizeof(float4), hipMemcpyDeviceToHost, s->stream), "readback");
                 HIP_TRY(hipStreamSynchronize(s->stream), "readback");
+            }
+            const float m_unsup = index == 0 ? 1.0f / (float)s->opts.frame_count : 0.0f;
+            if (which == FG_DEBUG_MASK_LEVEL) {
                 if (n * 4 > dst_bytes) return fail(FG_ERR_INVALID_ARG, "destination too small");
-                for (size_t i = 0; i < n; i++) ((float*)dst)[i] = m[i].x;
+                for (size_t i = 0; i < n; i++) ((float*)dst)[i] = v[i].w < 0.0f ? m_unsup : v[i].w;
                 return FG_OK;
             }
-            std::vector<float4> g(n);
-            HIP_TRY(hipMemcpyAsync(g.data(), s->dbgG + l.off, n * 16, hipMemcpyDeviceToHost, s->stream), "readback");
-            HIP_TRY(hipStreamSynchronize(s->stream), "readback");
             if (n * 12 > dst_bytes) return fail(FG_ERR_INVALID_ARG, "destination too small");
             for (size_t i = 0; i < n; i++) {
-                ((float*)dst)[i * 3] = g[i].x;
-                ((float*)dst)[i * 3 + 1] = g[i].y;
-                ((float*)dst)[i * 3 + 2] = g[i].z;
+                ((float*)dst)[i * 3] = v[i].x < 0.0f ? -v[i].x - 1.0f : v[i].x;
+                ((float*)dst)[i * 3 + 1] = v[i].y;
+                ((float*)dst)[i * 3 + 2] = v[i].z;
             }
             return FG_OK;
         }
@@ -693,6 +840,47 @@ int fg_stack_debug_readback(fg_stack* s, int which, uint32_t index, void* dst, s
         default:
             return fail(FG_ERR_INVALID_ARG, "unknown debug buffer");
     }
+}
+
+int fg_stack_reset(fg_stack* s) {
+    if (!s) return fail(FG_ERR_INVALID_ARG, "null stack");
+    int rc = set_device(s->ctx->device);
+    if (rc) return rc;
+    const size_t px = (size_t)s->W * s->H;
+    HIP_TRY(fg::launch_zero(s->S, px * sizeof(float), s->stream), "reset");
+    HIP_TRY(fg::launch_zero(s->U, px, s->stream), "reset");
+    HIP_TRY(fg::launch_zero(s->acc, s->lt.total * 3 * sizeof(float), s->stream), "reset");
+    if (s->best) HIP_TRY(fg::launch_fill_f32(s->best, px, -INFINITY, s->stream), "reset");
+    s->added = 0;
+    s->p1_done = 0;
+    s->pass2_done = 0;
+    s->stage_used = 0;
+    s->finished = false;
+    s->have_src = false;
+    recycle_events(s);
+    return FG_OK;
+}
+
+int fg_stack_set_profiling(fg_stack* s, int enable) {
+    if (!s) return fail(FG_ERR_INVALID_ARG, "null stack");
+    s->profiling = enable != 0;
+    return FG_OK;
+}
+
+int fg_stack_phase_times(fg_stack* s, double* ms, uint32_t* launches) {
+    if (!s || !ms) return fail(FG_ERR_INVALID_ARG, "null argument");
+    HIP_TRY(hipStreamSynchronize(s->stream), "sync");
+    for (int k = 0; k < FG_PHASE_COUNT; k++) {
+        double t = 0.0;
+        for (auto& pr : s->ev[k]) {
+            float f = 0.0f;
+            HIP_TRY(hipEventElapsedTime(&f, pr.first, pr.second), "hipEventElapsedTime");
+            t += f;
+        }
+        ms[k] = t;
+        if (launches) launches[k] = (uint32_t)s->ev[k].size();
+    }
+    return FG_OK;
 }
 
 int fg_fuse_stack(fg_context* ctx, uint32_t n, const fg_image_desc* descs, const void* const* pxs,
@@ -712,6 +900,7 @@ int fg_fuse_stack(fg_context* ctx, uint32_t n, const fg_image_desc* descs, const
     fg_stack_destroy(s);
     return rc;
 }
+
 
 // computeCommonOverlapRoi src/remap.zig:585-633 with transformPoint src/optimize.zig:3506-3525.
 static void transform_point(const fg_pose& p, double x, double y, uint32_t w, uint32_t h, double* ox, double* oy);

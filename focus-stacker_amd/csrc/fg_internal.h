@@ -55,38 +55,54 @@ void make_remap_params(const double pose[15], uint32_t w, uint32_t h, int32_t ro
 // ---- launchers (all asynchronous on `s`) ----
 hipError_t launch_remap(const void* src, int sample_bytes, int channels, uint32_t sw, uint32_t sh,
                         const RemapParams& p, uint32_t ow, uint32_t oh, void* dst, hipStream_t s);
-// RGB(A) aligned frame -> RGBA with alpha = max when the frame has no alpha
-hipError_t launch_to_rgba(const void* src, int sample_bytes, int extra, uint32_t w, uint32_t h, void* dst,
-                          hipStream_t s);
+// One frame of a stack as pass 1 sees it.
+struct FrameDesc {
+    const void* src;   // device pixels
+    uint32_t sw, sh;   // source size
+    int32_t kind;      // 0: rgb source remapped under p into the stack's ROI; 1: aligned frame
+    int32_t extra;     // aligned frame carries an alpha channel
+    RemapParams p;
+};
 
+// Pass 1 over frames [k0, k1): remap (or load) each frame into the RGBA cache, contrast
+// weight * alpha into the weight cache, and norm sum / union support / hybrid winners.
 struct Pass1Args {
-    const void* rgba;   // W*H*4 samples
+    const FrameDesc* frames;  // device array indexed by frame
+    int k0, k1;
+    int slot_base;            // frame index held in cache slot 0
     int sample_bytes;
-    uint32_t w, h;
-    uint32_t window;
-    float* weights;     // out: contrast * alpha
-    float* norm_sum;    // S += w
-    uint8_t* union_sup; // U |= alpha > 0
-    float* best;        // hybrid: winner weight (nullptr if not hybrid)
-    void* hard;         // hybrid: winner RGB
+    uint32_t w, h, window;
+    void* rgba;               // cache slot j at rgba + j * w*h*4 samples
+    float* weights;           // cache slot j at weights + j * w*h
+    float* norm_sum;          // nullptr: weights only (recompute in an uncached second pass)
+    uint8_t* union_sup;
+    float* best;              // hybrid only, else nullptr
+    void* hard;
 };
 hipError_t launch_pass1(const Pass1Args& a, hipStream_t s);
 
+// Pass 2 over frames [k0, k1) for every level: mask and gaussian pyramids, laplacian
+// accumulation into the planar accumulators.
 struct Pass2Args {
-    const void* rgba;        // level-0 gaussian (RGB) and image support (alpha > 0)
     int sample_bytes;
-    const float* weights;    // level-0 mask numerator
+    const LevelTable* lt;
+    int k0, k1;
+    int slot_base;            // frame held in slot 0 of the rgba / weight / level caches
+    const void* rgba;
+    const float* weights;
     const float* norm_sum;
     const uint8_t* union_sup;
-    uint32_t frame_count;
-    const LevelTable* lt;
-    float4* G;               // levels >= 1: rgb + support
-    float2* M;               // levels >= 1: mask + support
-    float* acc;              // planar accumulators: level l at 3*lv[l].off, planes R,G,B of lw*lh
+    uint32_t frame_count;     // N of normalizeWeightsInto's 1/N fallback
+    float4* lev;              // per-frame levels >= 1 (rgb + mask, support flags in signs)
+    size_t lev_stride;        // float4s per frame slot
+    float* acc;
+    float* contrib;           // scratch for the ordered sum of small levels
+    size_t contrib_floats;
 };
-hipError_t launch_pass2_frame(const Pass2Args& a, hipStream_t s);
-// debug: level-0 gaussian/mask of a frame into G[0]/M[0] slots of the debug buffers
-hipError_t launch_level0_debug(const Pass2Args& a, float4* G0, float2* M0, hipStream_t s);
+hipError_t launch_pass2(const Pass2Args& a, hipStream_t s);
+size_t pass2_contrib_floats(const LevelTable* lt, uint32_t frames);
+// level-0 gaussian + mask of slot j, encoded as the level buffers are
+hipError_t launch_level0_encode(const Pass2Args& a, int slot, float4* dst, hipStream_t s);
 
 hipError_t launch_collapse(float* acc, const LevelTable* lt, hipStream_t s);
 hipError_t launch_quantize(const float* acc0, uint32_t w, uint32_t h, int sample_bytes, void* out, hipStream_t s);

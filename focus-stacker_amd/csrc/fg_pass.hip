@@ -1,0 +1,702 @@
+// Pass 1 and pass 2 of a stack as frame-grouped tile kernels (gfx950, wave64, 256 threads).
+//
+// Pass 1 (k_pass1): for each 64x16 output tile, frames k0..k1-1 in order: the tile plus the
+// contrast halo is remapped (f64) into LDS, the RGBA tile goes to the frame cache, gray and the
+// local-contrast standard deviation are formed from LDS with f64 column sums, and the weight
+// (contrast * alpha) goes to the weight cache.  The norm sum, union support and hybrid winners
+// of the tile stay in registers across the frames and are written once.
+//
+// Pass 2 (k_pass2): for each level l and each tile of level l, frames k0..k1-1 in order: the
+// level-l region (tile + 4-pixel halo) is loaded into LDS, the support-aware 5x5 reduce builds
+// the level-(l+1) gaussian and mask of the tile (+1 halo) in LDS and writes their interior for
+// the next level, and the laplacian G[l] - EXPAND(G[l+1]) times the mask is added to the
+// accumulator held in registers.  The accumulator tile is read and written once per launch.
+// Levels too small to fill the GPU with tiles run one frame per block (grid.z) and write each
+// frame's product to scratch; k_ordered_sum then adds them to the accumulator in frame order.
+//
+// Every f32 value is rounded where the reference rounds and sums keep its order (frame order
+// for the norm sum and the accumulators, ky-then-kx order in the reduce), so results match the
+// reference's single-threaded arithmetic; see fg_device.h for the expand and remap formulas.
+#include "fg_device.h"
+
+namespace fg {
+
+// ---------------------------------------------------------------------------
+// packed RGBA samples in LDS
+// ---------------------------------------------------------------------------
+
+template <typename T>
+struct Pack;
+template <>
+struct Pack<uint8_t> {
+    using type = uint32_t;
+    __device__ static __forceinline__ type make(uint32_t r, uint32_t g, uint32_t b, uint32_t a) {
+        return r | (g << 8) | (b << 16) | (a << 24);
+    }
+    __device__ static __forceinline__ uint32_t ch(type v, int c) { return (v >> (8 * c)) & 0xffu; }
+};
+template <>
+struct Pack<uint16_t> {
+    using type = uint2;
+    __device__ static __forceinline__ type make(uint32_t r, uint32_t g, uint32_t b, uint32_t a) {
+        return make_uint2(r | (g << 16), b | (a << 16));
+    }
+    __device__ static __forceinline__ uint32_t ch(type v, int c) {
+        return c < 2 ? ((v.x >> (16 * c)) & 0xffffu) : ((v.y >> (16 * (c - 2))) & 0xffffu);
+    }
+};
+
+// RGBA of ROI pixel (x, y) of a frame: remapped source (remapRigidImage) or aligned frame
+// (alpha = full support when the frame has none, which leaves applySupportInto a no-op).
+template <typename T>
+__device__ __forceinline__ void frame_rgba(const FrameDesc& f, int x, int y, uint32_t o[4]) {
+    if (f.kind == 0) {
+        double sx, sy;
+        if (map_pixel(f.p, x, y, &sx, &sy)) {
+            sample_bilinear<T, 3>((const T*)f.src, (int)f.sw, (int)f.sh, sx, sy, o);
+        } else {
+            o[0] = o[1] = o[2] = o[3] = 0;
+        }
+    } else {
+        const int st = 3 + f.extra;
+        const T* q = (const T*)f.src + ((size_t)y * f.sw + x) * st;
+        o[0] = q[0];
+        o[1] = q[1];
+        o[2] = q[2];
+        o[3] = f.extra ? (uint32_t)q[3] : SampleMax<T>::i;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// pass 1
+// ---------------------------------------------------------------------------
+
+constexpr int P1W = 64, P1H = 16;
+
+size_t pass1_lds_bytes(int window, int sb) {
+    const int b = window / 2;
+    const size_t cols = P1W + 2 * b, rows = P1H + 2 * b;
+    return P1H * cols * (8 + 8 + 4) + rows * cols * ((sb == 1 ? 4 : 8) + 4);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_pass1(Pass1Args a) {
+    using P = Pack<T>;
+    using PT = typename P::type;
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int b = (int)a.window / 2;
+    const int cols = P1W + 2 * b, rows = P1H + 2 * b;
+    const int n_reg = rows * cols, n_col = P1H * cols;
+    double* csum = (double*)smem;
+    double* csq = csum + n_col;
+    PT* px = (PT*)(csq + n_col);
+    float* gray = (float*)(px + n_reg);
+    int* ccnt = (int*)(gray + n_reg);
+
+    const int W = (int)a.w, H = (int)a.h;
+    const int X0 = blockIdx.x * P1W, Y0 = blockIdx.y * P1H;
+    const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+    const bool contrast_ok = W >= (int)a.window && H >= (int)a.window;
+    const size_t fpx = (size_t)W * H;
+    const bool acc = a.norm_sum != nullptr;
+    const bool hyb = a.best != nullptr;
+
+    float s_acc[4], b_acc[4];
+    uint32_t u_acc[4], h_acc[4][3];
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        const int x = X0 + tx, y = Y0 + ty + 4 * j;
+        s_acc[j] = 0.0f;
+        u_acc[j] = 0;
+        b_acc[j] = 0.0f;
+        h_acc[j][0] = h_acc[j][1] = h_acc[j][2] = 0;
+        if (x < W && y < H) {
+            const size_t i = (size_t)y * W + x;
+            if (acc) {
+                s_acc[j] = a.norm_sum[i];
+                u_acc[j] = a.union_sup[i];
+            }
+            if (hyb) {
+                const T* hd = (const T*)a.hard + i * 3;
+                b_acc[j] = a.best[i];
+                h_acc[j][0] = hd[0];
+                h_acc[j][1] = hd[1];
+                h_acc[j][2] = hd[2];
+            }
+        }
+    }
+
+    for (int k = a.k0; k < a.k1; k++) {
+        const FrameDesc& f = a.frames[k];
+        const size_t slot = (size_t)(k - a.slot_base);
+        T* rgba = (T*)a.rgba + slot * fpx * 4;
+        float* wout = a.weights + slot * fpx;
+        // remap the tile and its contrast halo; gray < 0 marks unsupported or outside
+        for (int idx = threadIdx.x; idx < n_reg; idx += 256) {
+            const int gx = X0 - b + idx % cols, gy = Y0 - b + idx / cols;
+            uint32_t o[4] = {0, 0, 0, 0};
+            float g = -1.0f;
+            if (gx >= 0 && gy >= 0 && gx < W && gy < H) {
+                frame_rgba<T>(f, gx, gy, o);
+                if (o[3] > 0) g = (float)(o[0] + o[1] + o[2]) / SampleMax<T>::gray3;
+            }
+            px[idx] = P::make(o[0], o[1], o[2], o[3]);
+            gray[idx] = g;
+        }
+        __syncthreads();
+        // column sums of the window rows: f64 sums of x and of the f32 square x*x
+        // (computeRange src/fuse/contrast.zig:201-217)
+        if (contrast_ok) {
+            for (int idx = threadIdx.x; idx < n_col; idx += 256) {
+                const int r = idx / cols, c = idx % cols;
+                double s = 0.0, q = 0.0;
+                int n = 0;
+                for (int dy = 0; dy <= 2 * b; dy++) {
+                    const float v = gray[(r + dy) * cols + c];
+                    if (v >= 0.0f) {
+                        s += (double)v;
+                        q += (double)(v * v);
+                        n++;
+                    }
+                }
+                csum[idx] = s;
+                csq[idx] = q;
+                ccnt[idx] = n;
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const int r = ty + 4 * j;
+            const int x = X0 + tx, y = Y0 + r;
+            if (x >= W || y >= H) continue;
+            const size_t i = (size_t)y * W + x;
+            const PT c = px[(r + b) * cols + tx + b];
+            if (sizeof(T) == 1)
+                ((uint32_t*)rgba)[i] = *(const uint32_t*)&c;
+            else
+                ((uint2*)rgba)[i] = *(const uint2*)&c;
+            float stdw = 0.0f;
+            if (contrast_ok && x >= b && x < W - b && y >= b && y < H - b && gray[(r + b) * cols + tx + b] >= 0.0f) {
+                double s = 0.0, q = 0.0;
+                int n = 0;
+                for (int dx = 0; dx <= 2 * b; dx++) {
+                    const int ci = r * cols + tx + dx;
+                    s += csum[ci];
+                    q += csq[ci];
+                    n += ccnt[ci];
+                }
+                if (n > 1) {  // sampleStdDev src/fuse/contrast.zig:284-290
+                    const double denom = (double)(n - 1), nf = (double)n;
+                    double var = (q - ((s * s) / nf)) / denom;
+                    if (!(var > 0.0)) var = 0.0;
+                    stdw = (float)sqrt(var);
+                }
+            }
+            const uint32_t alpha = P::ch(c, 3);
+            const float wv = stdw * ((float)alpha / SampleMax<T>::f);  // applySupportInto
+            wout[i] = wv;
+            if (acc) {
+                s_acc[j] = s_acc[j] + wv;
+                if (alpha > 0) u_acc[j] = 1;
+            }
+            if (hyb && wv > b_acc[j]) {  // updateRange: strictly greater wins, first frame on ties
+                b_acc[j] = wv;
+                h_acc[j][0] = P::ch(c, 0);
+                h_acc[j][1] = P::ch(c, 1);
+                h_acc[j][2] = P::ch(c, 2);
+            }
+        }
+        __syncthreads();
+    }
+
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        const int x = X0 + tx, y = Y0 + ty + 4 * j;
+        if (x >= W || y >= H) continue;
+        const size_t i = (size_t)y * W + x;
+        if (acc) {
+            a.norm_sum[i] = s_acc[j];
+            a.union_sup[i] = (uint8_t)u_acc[j];
+        }
+        if (hyb) {
+            T* hd = (T*)a.hard + i * 3;
+            a.best[i] = b_acc[j];
+            hd[0] = (T)h_acc[j][0];
+            hd[1] = (T)h_acc[j][1];
+            hd[2] = (T)h_acc[j][2];
+        }
+    }
+}
+
+hipError_t launch_pass1(const Pass1Args& a, hipStream_t s) {
+    if (a.k1 <= a.k0) return hipSuccess;
+    const size_t lds = pass1_lds_bytes((int)a.window, a.sample_bytes);
+    dim3 grid((a.w + P1W - 1) / P1W, (a.h + P1H - 1) / P1H);
+    if (a.sample_bytes == 1) {
+        if (lds > 65536) {
+            hipError_t e = hipFuncSetAttribute((const void*)k_pass1<uint8_t>,
+                                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+            if (e != hipSuccess) return e;
+        }
+        hipLaunchKernelGGL(k_pass1<uint8_t>, grid, dim3(256), lds, s, a);
+    } else {
+        if (lds > 65536) {
+            hipError_t e = hipFuncSetAttribute((const void*)k_pass1<uint16_t>,
+                                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+            if (e != hipSuccess) return e;
+        }
+        hipLaunchKernelGGL(k_pass1<uint16_t>, grid, dim3(256), lds, s, a);
+    }
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// pass 2
+// ---------------------------------------------------------------------------
+// Level data in LDS and in the per-frame level buffers is one float4 per pixel:
+//   x = red,   or -(red + 1) when the pixel has no image support
+//   y, z = green, blue
+//   w = mask,  or -1 when the pixel has no mask support (level 0: no frame has alpha > 0,
+//              where the mask is exactly 1/N since every weight there is 0)
+// Unsupported pixels above level 0 have rgb 0, so -(0 + 1) = -1 decodes to 0.
+
+__device__ __forceinline__ float dec_r(float r) { return r < 0.0f ? -r - 1.0f : r; }
+
+struct LevelArgs {
+    int w, h;          // level l
+    int nw, nh;        // level l + 1 (unused at the top level)
+    int top;           // l == L - 1: no reduce, no expand
+    size_t lev_off;    // level l within a frame's level buffer (levels >= 1)
+    size_t nlev_off;   // level l + 1 within a frame's level buffer
+    size_t acc_off;    // level l within the planar accumulators
+};
+
+template <int TW, int TH>
+struct Tile {
+    static constexpr int RW = TW + 8, RH = TH + 7;          // level-l region: tile + 4 left/top, + 3 right/bottom
+                                                            // (rows padded to an even length for 8-byte LDS reads)
+    static constexpr int NW = TW / 2 + 2, NH = TH / 2 + 2;  // level-(l+1) region: tile/2 + 1 on each side
+    static constexpr int PPT = TW * TH / 256;
+    static constexpr int STEP = 256 / TW;
+};
+
+// Divisor of expandRgbExactNoWrap (src/fuse/pyramid.zig:1167-1354) for output (2i+a, 2j+b) of a
+// source level sw x sh: the in-bounds kernel weight, except where the reference divides by the
+// constants of the wider case (one-pixel-wide or one-pixel-tall source levels).
+__device__ __forceinline__ float nowrap_div(int i, int j, bool ev_x, bool ev_y, int sw, int sh, int y) {
+    const float hw = ev_x ? (float)((i > 0 ? 1 : 0) + 6 + (i + 1 < sw ? 1 : 0)) : (float)(4 + (i + 1 < sw ? 4 : 0));
+    const float vw = ev_y ? (float)((j > 0 ? 1 : 0) + 6 + (j + 1 < sh ? 1 : 0)) : (float)(1 + (j + 1 < sh ? 1 : 0));
+    float div = hw * vw;
+    if (sw == 1 && sh > 1 && j <= sh - 2 && ev_x) div = ev_y ? (j == 0 ? 49.0f : 56.0f) : 14.0f;
+    if (sh == 1 && sw > 1 && y == 0 && i == sw - 1) div = ev_x ? 49.0f : 28.0f;
+    return div;
+}
+
+// Running sums of one support-aware reduce output (reduceRgbWithSupportRowsSequential /
+// reduceScalarWithSupportRowsSequential, src/fuse/pyramid.zig:997-1099).
+struct RedAcc {
+    float s0 = 0.0f, s1 = 0.0f, s2 = 0.0f, ws = 0.0f, mv = 0.0f, mws = 0.0f;
+};
+
+__device__ __forceinline__ float kw_of(int r) { return r == 2 ? 6.0f : ((r == 1 || r == 3) ? 4.0f : 1.0f); }
+
+// Adds the five taps of one LDS row (columns base..base+4, kx order) to the outputs it feeds, with
+// vertical kernel weights wya / wyb.  A tap without support adds value * 0 (= +-0) and weight 0,
+// which leaves every sum unchanged, exactly as the reference's `continue`.
+template <bool DO_A, bool DO_B>
+__device__ __forceinline__ void reduce_row(const float* rR, const float* rG, const float* rB, const float* rM, int base,
+                                           float wya, float wyb, RedAcc& a, RedAcc& b) {
+    const float KW[5] = {1.0f, 4.0f, 6.0f, 4.0f, 1.0f};
+    float tr[5], tg[5], tb[5], tm[5];
+    {
+        const float2 u0 = *(const float2*)&rR[base], u1 = *(const float2*)&rR[base + 2];
+        tr[0] = u0.x; tr[1] = u0.y; tr[2] = u1.x; tr[3] = u1.y; tr[4] = rR[base + 4];
+    }
+    {
+        const float2 u0 = *(const float2*)&rG[base], u1 = *(const float2*)&rG[base + 2];
+        tg[0] = u0.x; tg[1] = u0.y; tg[2] = u1.x; tg[3] = u1.y; tg[4] = rG[base + 4];
+    }
+    {
+        const float2 u0 = *(const float2*)&rB[base], u1 = *(const float2*)&rB[base + 2];
+        tb[0] = u0.x; tb[1] = u0.y; tb[2] = u1.x; tb[3] = u1.y; tb[4] = rB[base + 4];
+    }
+    {
+        const float2 u0 = *(const float2*)&rM[base], u1 = *(const float2*)&rM[base + 2];
+        tm[0] = u0.x; tm[1] = u0.y; tm[2] = u1.x; tm[3] = u1.y; tm[4] = rM[base + 4];
+    }
+#pragma unroll
+    for (int kx = 0; kx < 5; kx++) {
+        const bool gs = tr[kx] >= 0.0f, ms = tm[kx] >= 0.0f;
+        if (DO_A) {
+            const float w = wya * KW[kx];
+            const float gw = gs ? w : 0.0f, mw = ms ? w : 0.0f;
+            a.s0 += tr[kx] * gw;
+            a.s1 += tg[kx] * gw;
+            a.s2 += tb[kx] * gw;
+            a.ws += gw;
+            a.mv += tm[kx] * mw;
+            a.mws += mw;
+        }
+        if (DO_B) {
+            const float w = wyb * KW[kx];
+            const float gw = gs ? w : 0.0f, mw = ms ? w : 0.0f;
+            b.s0 += tr[kx] * gw;
+            b.s1 += tg[kx] * gw;
+            b.s2 += tb[kx] * gw;
+            b.ws += gw;
+            b.mv += tm[kx] * mw;
+            b.mws += mw;
+        }
+    }
+}
+
+template <typename T, bool L0, int TW, int TH, bool SPLIT>
+__global__ void __launch_bounds__(256) k_pass2(Pass2Args a, LevelArgs g) {
+    using TL = Tile<TW, TH>;
+    constexpr int RW = TL::RW, RH = TL::RH, NW = TL::NW, NH = TL::NH;
+    // level-l region, planar: red (support flag in the sign), green, blue, mask (flag in the sign)
+    __shared__ __align__(16) float rR[RH * RW];
+    __shared__ __align__(16) float rG[RH * RW];
+    __shared__ __align__(16) float rB[RH * RW];
+    __shared__ __align__(16) float rM[RH * RW];
+    __shared__ float sS[L0 ? RH * RW : 1];        // norm sum, -1 where no frame has support
+    __shared__ float nR[NH * NW], nG[NH * NW], nB[NH * NW];   // level-(l+1) gaussian (values)
+    __shared__ float hR[NH * TW], hG[NH * TW], hB[NH * TW];   // its horizontal expansion per tile column
+
+    const int X0 = blockIdx.x * TW, Y0 = blockIdx.y * TH;
+    const int tx = threadIdx.x % TW, ty = threadIdx.x / TW;
+    const int n = g.w * g.h;
+    float* accp = a.acc + g.acc_off;
+    const float m_unsup = L0 ? 1.0f / (float)a.frame_count : 0.0f;
+    const int kb = SPLIT ? a.k0 + (int)blockIdx.z : a.k0;
+    const int ke = SPLIT ? kb + 1 : a.k1;
+    const bool clamp_regime = (g.w > g.h ? g.w : g.h) > 512;  // exact_expand_max_dim, src/fuse/pyramid.zig:8
+    const int ox1 = X0 / 2 - 1, oy1 = Y0 / 2 - 1;
+
+    float ar[TL::PPT], ag[TL::PPT], ab[TL::PPT];
+#pragma unroll
+    for (int j = 0; j < TL::PPT; j++) {
+        ar[j] = ag[j] = ab[j] = 0.0f;
+        const int x = X0 + tx, y = Y0 + ty + j * TL::STEP;
+        if (!SPLIT && x < g.w && y < g.h) {
+            const int i = y * g.w + x;
+            ar[j] = accp[i];
+            ag[j] = accp[i + n];
+            ab[j] = accp[i + 2 * n];
+        }
+    }
+    if (L0) {
+#pragma unroll 1
+        for (int idx = threadIdx.x; idx < RH * RW; idx += 256) {
+            const int gx = X0 - 4 + idx % RW, gy = Y0 - 4 + idx / RW;
+            float v = -1.0f;
+            if (gx >= 0 && gy >= 0 && gx < g.w && gy < g.h) {
+                const int i = gy * g.w + gx;
+                if (a.union_sup[i]) v = a.norm_sum[i];
+            }
+            sS[idx] = v;
+        }
+    }
+
+    for (int k = kb; k < ke; k++) {
+        const size_t slot = (size_t)(k - a.slot_base);
+        // A: level-l region of frame k
+        {
+            const T* rg = L0 ? (const T*)a.rgba + slot * (size_t)n * 4 : nullptr;
+            const float* wts = L0 ? a.weights + slot * (size_t)n : nullptr;
+            const float4* lv = L0 ? nullptr : a.lev + slot * a.lev_stride + g.lev_off;
+#pragma unroll 2
+            for (int idx = threadIdx.x; idx < RH * RW; idx += 256) {
+                const int gx = X0 - 4 + idx % RW, gy = Y0 - 4 + idx / RW;
+                float4 v = make_float4(-1.0f, 0.0f, 0.0f, -1.0f);
+                if (gx >= 0 && gy >= 0 && gx < g.w && gy < g.h) {
+                    const int i = gy * g.w + gx;
+                    if (L0) {
+                        // fillRgbBase + fillBinarySupport (src/fuse/pyramid.zig:502-503) and
+                        // normalizeWeightsInto (:303-316) with the union support (:202)
+                        const T* q = rg + (size_t)i * 4;
+                        const float r = (float)q[0];
+                        v.x = q[3] > 0 ? r : -(r + 1.0f);
+                        v.y = (float)q[1];
+                        v.z = (float)q[2];
+                        const float sv = sS[idx];
+                        v.w = sv < 0.0f ? -1.0f : normalized_mask(wts[i], sv, m_unsup);
+                    } else {
+                        v = lv[i];
+                    }
+                }
+                rR[idx] = v.x;
+                rG[idx] = v.y;
+                rB[idx] = v.z;
+                rM[idx] = v.w;
+            }
+        }
+        __syncthreads();
+        if (!g.top) {
+            // B: reduceRgbWithSupport / reduceScalarWithSupport (src/fuse/pyramid.zig:997-1099),
+            // two vertically adjacent outputs per thread.  A tap without support adds
+            // value * 0 (= +-0) and weight 0, which leaves every sum unchanged, exactly as the
+            // reference's `continue`.
+            constexpr int NP = NW * (NH / 2);
+            float4* lvn = a.lev + slot * a.lev_stride + g.nlev_off;
+            for (int p = threadIdx.x; p < NP; p += 256) {
+                const int oi = p % NW, oj = 2 * (p / NW);
+                RedAcc ra, rb;
+                const int base0 = (2 * oj) * RW + 2 * oi;
+                // rows 0-1 feed the upper output only, rows 2-4 both, rows 5-6 the lower only
+#pragma unroll 1
+                for (int rr = 0; rr < 2; rr++)
+                    reduce_row<true, false>(rR, rG, rB, rM, base0 + rr * RW, kw_of(rr), 0.0f, ra, rb);
+#pragma unroll 1
+                for (int rr = 2; rr < 5; rr++)
+                    reduce_row<true, true>(rR, rG, rB, rM, base0 + rr * RW, kw_of(rr), kw_of(rr - 2), ra, rb);
+#pragma unroll 1
+                for (int rr = 5; rr < 7; rr++)
+                    reduce_row<false, true>(rR, rG, rB, rM, base0 + rr * RW, 0.0f, kw_of(rr - 2), ra, rb);
+                const float a0 = ra.s0, a1 = ra.s1, a2 = ra.s2, aw = ra.ws, am = ra.mv, amw = ra.mws;
+                const float b0 = rb.s0, b1 = rb.s1, b2 = rb.s2, bw = rb.ws, bm = rb.mv, bmw = rb.mws;
+                const int gi = ox1 + oi;
+#pragma unroll
+                for (int h2 = 0; h2 < 2; h2++) {
+                    const int o = (oj + h2) * NW + oi;
+                    const int gj = oy1 + oj + h2;
+                    const float s0 = h2 ? b0 : a0, s1 = h2 ? b1 : a1, s2 = h2 ? b2 : a2, ws = h2 ? bw : aw;
+                    const float mv = h2 ? bm : am, mws = h2 ? bmw : amw;
+                    float e0 = 0.0f, e1 = 0.0f, e2 = 0.0f;
+                    if (gi >= 0 && gj >= 0 && gi < g.nw && gj < g.nh) {
+                        float4 outv = make_float4(-1.0f, 0.0f, 0.0f, 0.0f);
+                        if (ws > 0.0f) {
+                            e0 = s0 / ws;
+                            e1 = s1 / ws;
+                            e2 = s2 / ws;
+                            outv = make_float4(e0, e1, e2, 0.0f);
+                        }
+                        outv.w = mws > 0.0f ? mv / mws : -1.0f;
+                        if (oi >= 1 && oi <= TW / 2 && oj + h2 >= 1 && oj + h2 <= TH / 2)
+                            lvn[gj * g.nw + gi] = outv;
+                    }
+                    nR[o] = e0;
+                    nG[o] = e1;
+                    nB[o] = e2;
+                }
+            }
+            __syncthreads();
+            // C: horizontal pass of EXPAND for every level-(l+1) row of the region and tile column
+            // (horizontalExpandRgbRow :1748-1775 above 512 px; the running sums of
+            // expandRgbExactNoWrap :1197-1214 at or below 512 px, zero outside the level)
+#pragma unroll 1
+            for (int idx = threadIdx.x; idx < NH * TW; idx += 256) {
+                const int r = idx / TW, x = X0 + idx % TW;
+                const int gj = oy1 + r;
+                float h0 = 0.0f, h1 = 0.0f, h2 = 0.0f;
+                if (x < g.w && gj >= 0 && gj < g.nh) {
+                    const int i = x >> 1;
+                    const int rb = r * NW - ox1;
+                    if (clamp_regime) {
+                        if ((x & 1) == 0) {
+                            const int c0 = rb + clamp_coord(i - 1, g.nw), c1 = rb + clamp_coord(i, g.nw),
+                                      c2 = rb + clamp_coord(i + 1, g.nw);
+                            h0 = ((nR[c0] + 6.0f * nR[c1]) + nR[c2]) * 0.125f;
+                            h1 = ((nG[c0] + 6.0f * nG[c1]) + nG[c2]) * 0.125f;
+                            h2 = ((nB[c0] + 6.0f * nB[c1]) + nB[c2]) * 0.125f;
+                        } else {
+                            const int c0 = rb + clamp_coord(i, g.nw), c1 = rb + clamp_coord(i + 1, g.nw);
+                            h0 = (nR[c0] + nR[c1]) * 0.5f;
+                            h1 = (nG[c0] + nG[c1]) * 0.5f;
+                            h2 = (nB[c0] + nB[c1]) * 0.5f;
+                        }
+                    } else {
+                        const bool l_in = i - 1 >= 0, r_in = i + 1 < g.nw;
+                        const int c1 = rb + i;
+                        if ((x & 1) == 0) {
+                            const float z0 = l_in ? nR[c1 - 1] : 0.0f, z2 = r_in ? nR[c1 + 1] : 0.0f;
+                            const float y0 = l_in ? nG[c1 - 1] : 0.0f, y2 = r_in ? nG[c1 + 1] : 0.0f;
+                            const float w0 = l_in ? nB[c1 - 1] : 0.0f, w2 = r_in ? nB[c1 + 1] : 0.0f;
+                            h0 = (z0 + 6.0f * nR[c1]) + z2;
+                            h1 = (y0 + 6.0f * nG[c1]) + y2;
+                            h2 = (w0 + 6.0f * nB[c1]) + w2;
+                        } else {
+                            const float z2 = r_in ? nR[c1 + 1] : 0.0f;
+                            const float y2 = r_in ? nG[c1 + 1] : 0.0f;
+                            const float w2 = r_in ? nB[c1 + 1] : 0.0f;
+                            h0 = 4.0f * (nR[c1] + z2);
+                            h1 = 4.0f * (nG[c1] + y2);
+                            h2 = 4.0f * (nB[c1] + w2);
+                        }
+                    }
+                }
+                hR[idx] = h0;
+                hG[idx] = h1;
+                hB[idx] = h2;
+            }
+            __syncthreads();
+        }
+        // D: vertical pass of EXPAND, laplacian, accumulate
+        // acc[l] += (G[l] - EXPAND(G[l+1])) * M[l]  (accumulateLapRowsSequential :1506-1531),
+        // top level acc += G * M (accumulateGaussianRowsSequential :1533-1554)
+#pragma unroll 1
+        for (int j = 0; j < TL::PPT; j++) {
+            const int x = X0 + tx, y = Y0 + ty + j * TL::STEP;
+            if (x >= g.w || y >= g.h) continue;
+            const int ci = (y - Y0 + 4) * RW + (x - X0 + 4);
+            f3 lap = mk3(dec_r(rR[ci]), rG[ci], rB[ci]);
+            const float mraw = rM[ci];
+            const float m = mraw < 0.0f ? m_unsup : mraw;
+            if (!g.top) {
+                const int jj = y >> 1;
+                const int xx = x - X0;
+                f3 e;
+                if (clamp_regime) {
+                    if ((y & 1) == 0) {
+                        const int q0 = (clamp_coord(jj - 1, g.nh) - oy1) * TW + xx;
+                        const int q1 = (clamp_coord(jj, g.nh) - oy1) * TW + xx;
+                        const int q2 = (clamp_coord(jj + 1, g.nh) - oy1) * TW + xx;
+                        e = mk3(((hR[q0] + 6.0f * hR[q1]) + hR[q2]) * 0.125f,
+                                ((hG[q0] + 6.0f * hG[q1]) + hG[q2]) * 0.125f,
+                                ((hB[q0] + 6.0f * hB[q1]) + hB[q2]) * 0.125f);
+                    } else {
+                        const int q = (clamp_coord(jj, g.nh) - oy1) * TW + xx;
+                        e = mk3((hR[q] + hR[q]) * 0.5f, (hG[q] + hG[q]) * 0.5f, (hB[q] + hB[q]) * 0.5f);
+                    }
+                } else {
+                    const int q1 = (jj - oy1) * TW + xx;
+                    f3 num;
+                    if ((y & 1) == 0) {
+                        const int q0 = q1 - TW, q2 = q1 + TW;
+                        num = mk3((hR[q0] + 6.0f * hR[q1]) + hR[q2], (hG[q0] + 6.0f * hG[q1]) + hG[q2],
+                                  (hB[q0] + 6.0f * hB[q1]) + hB[q2]);
+                    } else {
+                        const int q2 = q1 + TW;
+                        num = mk3(hR[q1] + hR[q2], hG[q1] + hG[q2], hB[q1] + hB[q2]);
+                    }
+                    e = div3(num, nowrap_div(x >> 1, jj, (x & 1) == 0, (y & 1) == 0, g.nw, g.nh, y));
+                }
+                lap = sub3(lap, e);
+            }
+            if (SPLIT) {
+                const int i = y * g.w + x;
+                float* cp = a.contrib + (size_t)(k - a.k0) * 3 * (size_t)n;
+                cp[i] = lap.x * m;
+                cp[i + n] = lap.y * m;
+                cp[i + 2 * n] = lap.z * m;
+            } else {
+                ar[j] = ar[j] + lap.x * m;
+                ag[j] = ag[j] + lap.y * m;
+                ab[j] = ab[j] + lap.z * m;
+            }
+        }
+        __syncthreads();
+    }
+
+    if (!SPLIT) {
+#pragma unroll
+        for (int j = 0; j < TL::PPT; j++) {
+            const int x = X0 + tx, y = Y0 + ty + j * TL::STEP;
+            if (x >= g.w || y >= g.h) continue;
+            const int i = y * g.w + x;
+            accp[i] = ar[j];
+            accp[i + n] = ag[j];
+            accp[i + 2 * n] = ab[j];
+        }
+    }
+}
+
+// acc += contribution of each frame, in frame order
+__global__ void __launch_bounds__(256) k_ordered_sum(float* __restrict__ acc, const float* __restrict__ contrib,
+                                                     int nk, size_t n3) {
+    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n3) return;
+    float v = acc[i];
+    for (int k = 0; k < nk; k++) v = v + contrib[(size_t)k * n3 + i];
+    acc[i] = v;
+}
+
+constexpr int kMinTiles = 512;  // two tiles per CU before a level loops frames inside its blocks
+
+static int tiles(int w, int h, int tw, int th) { return ((w + tw - 1) / tw) * ((h + th - 1) / th); }
+
+size_t pass2_contrib_floats(const LevelTable* lt, uint32_t frames) {
+    size_t mx = 0;
+    for (int l = 0; l < lt->n; l++) {
+        const int w = (int)lt->lv[l].w, h = (int)lt->lv[l].h;
+        if (tiles(w, h, 64, 16) >= kMinTiles || tiles(w, h, 16, 16) >= kMinTiles) continue;
+        const size_t need = (size_t)3 * w * h * frames;
+        mx = need > mx ? need : mx;
+    }
+    return mx;
+}
+
+template <typename T, bool L0>
+static hipError_t launch_level(const Pass2Args& a, const LevelArgs& g, int nk, hipStream_t s) {
+    if (tiles(g.w, g.h, 64, 16) >= kMinTiles) {
+        dim3 grid((g.w + 63) / 64, (g.h + 15) / 16);
+        hipLaunchKernelGGL((k_pass2<T, L0, 64, 16, false>), grid, dim3(256), 0, s, a, g);
+    } else if (tiles(g.w, g.h, 16, 16) >= kMinTiles || nk == 1) {
+        dim3 grid((g.w + 15) / 16, (g.h + 15) / 16);
+        hipLaunchKernelGGL((k_pass2<T, L0, 16, 16, false>), grid, dim3(256), 0, s, a, g);
+    } else {
+        const size_t n3 = (size_t)3 * g.w * g.h;
+        if (n3 * (size_t)nk > a.contrib_floats) return hipErrorInvalidValue;
+        dim3 grid((g.w + 15) / 16, (g.h + 15) / 16, nk);
+        hipLaunchKernelGGL((k_pass2<T, L0, 16, 16, true>), grid, dim3(256), 0, s, a, g);
+        hipLaunchKernelGGL(k_ordered_sum, dim3((unsigned)((n3 + 255) / 256)), dim3(256), 0, s, a.acc + g.acc_off,
+                           (const float*)a.contrib, nk, n3);
+    }
+    return hipGetLastError();
+}
+
+template <typename T>
+static hipError_t pass2_t(const Pass2Args& a, hipStream_t s) {
+    const LevelTable& lt = *a.lt;
+    const int L = lt.n;
+    const int nk = a.k1 - a.k0;
+    if (nk <= 0) return hipSuccess;
+    const size_t base1 = L > 1 ? lt.lv[1].off : 0;
+    for (int l = 0; l < L; l++) {
+        LevelArgs g;
+        g.w = (int)lt.lv[l].w;
+        g.h = (int)lt.lv[l].h;
+        g.top = l == L - 1;
+        g.nw = g.top ? 0 : (int)lt.lv[l + 1].w;
+        g.nh = g.top ? 0 : (int)lt.lv[l + 1].h;
+        g.lev_off = l >= 1 ? lt.lv[l].off - base1 : 0;
+        g.nlev_off = g.top ? 0 : lt.lv[l + 1].off - base1;
+        g.acc_off = 3 * lt.lv[l].off;
+        hipError_t e = l == 0 ? launch_level<T, true>(a, g, nk, s) : launch_level<T, false>(a, g, nk, s);
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
+hipError_t launch_pass2(const Pass2Args& a, hipStream_t s) {
+    return a.sample_bytes == 1 ? pass2_t<uint8_t>(a, s) : pass2_t<uint16_t>(a, s);
+}
+
+// level-0 gaussian + mask of one cached frame in the level-buffer encoding (debug readback)
+template <typename T>
+__global__ void k_level0_encode(Pass2Args a, size_t n, int slot, float4* dst) {
+    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const T* q = (const T*)a.rgba + ((size_t)slot * n + i) * 4;
+    const float r = (float)q[0];
+    float4 v;
+    v.x = q[3] > 0 ? r : -(r + 1.0f);
+    v.y = (float)q[1];
+    v.z = (float)q[2];
+    const float def = 1.0f / (float)a.frame_count;
+    v.w = a.union_sup[i] ? normalized_mask(a.weights[(size_t)slot * n + i], a.norm_sum[i], def) : -1.0f;
+    dst[i] = v;
+}
+
+hipError_t launch_level0_encode(const Pass2Args& a, int slot, float4* dst, hipStream_t s) {
+    const size_t n = (size_t)a.lt->lv[0].w * a.lt->lv[0].h;
+    dim3 grid((unsigned)((n + 255) / 256));
+    if (a.sample_bytes == 1)
+        hipLaunchKernelGGL(k_level0_encode<uint8_t>, grid, dim3(256), 0, s, a, n, slot, dst);
+    else
+        hipLaunchKernelGGL(k_level0_encode<uint16_t>, grid, dim3(256), 0, s, a, n, slot, dst);
+    return hipGetLastError();
+}
+
+}  // namespace fg

@@ -114,6 +114,9 @@ def lib() -> C.CDLL:
     L.fg_stack_stream.argtypes = [vp]
     L.fg_stack_stream.restype = vp
     L.fg_stack_sync.argtypes = [vp]
+    L.fg_stack_reset.argtypes = [vp]
+    L.fg_stack_set_profiling.argtypes = [vp, i32]
+    L.fg_stack_phase_times.argtypes = [vp, vp, vp]
     L.fg_fuse_stack.argtypes = [vp, u32, vp, vp, vp, C.POINTER(Rect), C.POINTER(StackOpts), vp]
     _lib = L
     return L
@@ -248,6 +251,19 @@ class Stack:
             out = np.zeros((lh, lw) if which == "mask_level" else (lh, lw, 3), np.float32)
         check(lib().fg_stack_debug_readback(self.h_, code, index, _p(out), out.nbytes))
         return out
+
+    def reset(self):
+        check(lib().fg_stack_reset(self.h_))
+
+    def set_profiling(self, on: bool = True):
+        check(lib().fg_stack_set_profiling(self.h_, 1 if on else 0))
+
+    def phase_times(self) -> dict:
+        ms = np.zeros(4, np.float64)
+        n = np.zeros(4, np.uint32)
+        check(lib().fg_stack_phase_times(self.h_, _p(ms), _p(n)))
+        names = ("ingest", "pass1", "pass2", "finish")
+        return {k: (float(ms[i]), int(n[i])) for i, k in enumerate(names)}
 
     def stream(self) -> int:
         return lib().fg_stack_stream(self.h_) or 0

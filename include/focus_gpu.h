@@ -162,6 +162,16 @@ void* fg_stack_stream(const fg_stack* stack);
 /* Blocks until the stack's queued work is done. */
 int fg_stack_sync(fg_stack* stack);
 
+/* Starts a new stack of the same geometry and options on the same device buffers (no
+ * allocation): clears the norm sum, union support, accumulators and frame counters. */
+int fg_stack_reset(fg_stack* stack);
+
+/* Device time per phase, from HIP events recorded on the stack's stream while profiling is
+ * on (accumulated since the last reset).  ms[k] / launches[k] for k < FG_PHASE_COUNT. */
+enum fg_phase { FG_PHASE_INGEST = 0, FG_PHASE_PASS1 = 1, FG_PHASE_PASS2 = 2, FG_PHASE_FINISH = 3, FG_PHASE_COUNT = 4 };
+int fg_stack_set_profiling(fg_stack* stack, int enable);
+int fg_stack_phase_times(fg_stack* stack, double* ms, uint32_t* launches);
+
 /* One-shot: every frame through pass 1, pass 2 and collapse (poses may be NULL for
  * aligned frames, roi may be NULL). */
 int fg_fuse_stack(fg_context* ctx, uint32_t n, const fg_image_desc* descs, const void* const* pxs,
