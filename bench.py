@@ -7,7 +7,10 @@ remapped into the common-overlap ROI and fused with the reference's full pyramid
 Frames are generated on the GPU and stay resident in HBM; the timed region is K steps of
 {reset, 30 x add_frame_device (remap + pass 1), finish_device (pass 2 + collapse)}.
 
-Output: one JSON line (see DESIGN.md for the roofline byte model and the CPU baseline).
+Output: one JSON line.  roofline: the longer of the two dominant launches (k_pass1, the level-0
+k_pass2), its per frame-pixel algorithmic bytes (kernel_models) times the frame-pixels of one
+launch over its mean duration from HIP events on the stack's stream in the timed region.
+cpu_baseline: the single-threaded C oracle on a bounded slice of the same stack.
 """
 import argparse
 import json
@@ -28,6 +31,20 @@ def alg_bytes_per_frame_px(sample_bytes: int, n: int) -> float:
     """Two-pass streaming model: pass 1 = 3s + 14, pass 2 = 3s + 41, collapse (12*4/3 + 3s)/N."""
     s = sample_bytes
     return (3 * s + 14) + (3 * s + 41) + (16 + 3 * s) / n
+
+
+def kernel_models(s: int) -> dict:
+    """Per frame-pixel algorithmic bytes of the two dominant launches (s = sample bytes).
+    k_pass1: the pass-1 term, read the source (3s), write w (4), RMW the norm sum (8) and the
+    union support (2).  k_pass2_level0: the level-0 share of the pass-2 term, read the source
+    (3s), w (4), S (4), U (1) and RMW the level-0 accumulator (24)."""
+    return {"k_pass1": ("pass1", 3 * s + 14), "k_pass2_level0": ("pass2_level0", 3 * s + 33)}
+
+
+# HBM bytes per launch of the default workload, from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE
+# runs scaled by the same counters on a plain copy (profiles/pmc_traffic_30x7952x5304_u8.md)
+TRAFFIC = {"k_pass1": 20.036e9, "k_pass2_level0": 29.718e9}
+COPY_PROBE_BYTES = 256 << 20
 
 
 def pct(v, q):
@@ -66,8 +83,9 @@ def main():
     ap.add_argument("--profile-only", action="store_true", help="one profiled step (for rocprofv3 runs)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py: this build runs on one GPU; multi-GPU sharding is described in DESIGN.md "
-              "but not launched (--gpus must be 1)", file=sys.stderr)
+        print("bench.py: runs on one GPU only; a stack would shard across GPUs by frames with an "
+              "all-reduce of the norm sum and the accumulator pyramid, which is not built "
+              "(--gpus must be 1)", file=sys.stderr)
         sys.exit(2)
 
     import numpy as np
@@ -104,9 +122,15 @@ def main():
         stack.finish_device(out.data_ptr())
 
     if args.profile_only:
+        # a plain copy of known size first: the scale of FETCH_SIZE / WRITE_SIZE per byte moved
+        cp_src = torch.rand(COPY_PROBE_BYTES // 4, device=dev)
+        cp_dst = torch.empty_like(cp_src)
+        torch.mul(cp_src, 2.0, out=cp_dst)
+        torch.cuda.synchronize()
         step()
         stack.sync()
-        print(json.dumps({"profile_step": "done", "roi": roi, "levels": levels}), flush=True)
+        print(json.dumps({"profile_step": "done", "roi": roi, "levels": levels,
+                          "copy_probe_bytes": {"read": COPY_PROBE_BYTES, "write": COPY_PROBE_BYTES}}), flush=True)
         return
 
     print(f"bench: roi={roi} ({w}x{h}), {levels} levels, warmup {args.warmup}", file=sys.stderr, flush=True)
@@ -115,7 +139,10 @@ def main():
     stack.sync()
     torch.cuda.synchronize()
 
-    lat = []
+    # timed region: K steps; HIP events on the stack's stream bracket every step and every kernel
+    # group the library launches (fg_stack_set_profiling), so per-launch durations come from the
+    # same run as the wall-clock rate
+    stack.set_profiling(True)
     evs = []
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -129,33 +156,30 @@ def main():
     stack.sync()
     torch.cuda.synchronize()
     t1 = time.perf_counter()
+    ph = stack.phase_times()
+    stack.set_profiling(False)
     lat = [a.elapsed_time(b) for a, b in evs]
     ms_per_step = (t1 - t0) * 1000.0 / args.steps
     frame_px = n * w * h
     value = frame_px / (ms_per_step / 1000.0) / 1e6
-
-    # one profiled step: per-phase device time from HIP events on the stack's stream
-    stack.set_profiling(True)
-    step()
-    ph = stack.phase_times()
-    stack.set_profiling(False)
     print(f"bench: {ms_per_step:.2f} ms/step, {value:.1f} MPix/s, phases {ph}", file=sys.stderr, flush=True)
 
-    # roofline of the dominant phase with the per-unit algorithmic bytes of the terms it carries
     s_b = 1
-    phase_bytes = {  # bytes per frame-pixel (per stack-pixel for finish), see DESIGN.md
-        "ingest": 3 * s_b + 4 * s_b,          # remap: read RGB source, write the RGBA frame
-        "pass1": 4 * s_b + 4 + 8 + 2,        # read RGBA, write w, RMW S, RMW U
-        "pass2": 3 * s_b + 41,               # pass-2 term of the streaming model
-        "finish": 16 + 3 * s_b,              # collapse: read accumulators, write the image
-    }
-    dom = max(("ingest", "pass1", "pass2"), key=lambda k: ph[k][0])
-    launches = max(ph[dom][1], 1)
-    avg_ms = ph[dom][0] / launches
-    units = w * h
-    ach = phase_bytes[dom] * units / (avg_ms / 1000.0) / 1e9
+    kern = {}
+    for name, (phase, per_unit) in kernel_models(s_b).items():
+        total_ms, launches = ph[phase]
+        if launches == 0:
+            continue
+        avg_ms = total_ms / launches
+        units = n * w * h  # one launch covers every frame of the stack at its level
+        ach = per_unit * units / (avg_ms / 1000.0) / 1e9
+        kern[name] = {"avg_launch_ms": round(avg_ms, 4), "launches": launches, "bytes_per_unit": per_unit,
+                      "units_per_launch": units, "achieved": round(ach, 1), "frac": round(ach / PEAK_HBM_GBS, 4)}
+    dom = max(kern, key=lambda k: kern[k]["avg_launch_ms"])
     pipe_bytes = alg_bytes_per_frame_px(s_b, n) * frame_px
     pipe_gbs = pipe_bytes / (ms_per_step / 1000.0) / 1e9
+    cfg_name = f"{n} x {spec.width}x{spec.height} RGB u8"
+    traffic = TRAFFIC.get(dom) if (n, spec.width, spec.height, args.levels) == (30, 7952, 5304, 0) else None
 
     line = {
         "metric": METRIC,
@@ -170,16 +194,17 @@ def main():
         "vs_baseline": None,
         "dtype": "u8",
         "data": "synthetic",
-        "config": {"workload": f"{n} x 7952x5304 RGB u8 stack (synthetic, seeded), remap + pyramid-contrast, "
-                               f"{levels} levels", "frames": n, "source": [spec.width, spec.height],
-                   "roi": list(roi), "levels": levels, "parallelism": "single GPU"},
+        "config": {"workload": f"{cfg_name} stack (synthetic, seeded), remap + pyramid-contrast, {levels} levels",
+                   "frames": n, "source": [spec.width, spec.height], "roi": list(roi), "levels": levels,
+                   "parallelism": "single GPU"},
         "latency_ms": {"p50": round(pct(lat, 50), 3), "p99": round(pct(lat, 99), 3)},
-        "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(ach, 1), "peak": PEAK_HBM_GBS,
-                     "unit": "GB/s", "frac": round(ach / PEAK_HBM_GBS, 4), "traffic": None,
-                     "avg_launch_ms": round(avg_ms, 4)},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": kern[dom]["achieved"], "peak": PEAK_HBM_GBS,
+                     "unit": "GB/s", "frac": kern[dom]["frac"], "traffic": traffic,
+                     "avg_launch_ms": kern[dom]["avg_launch_ms"]},
+        "kernels": kern,
         "pipeline": {"alg_bytes_per_frame_px": round(alg_bytes_per_frame_px(s_b, n), 3),
                      "achieved_GBs": round(pipe_gbs, 1), "frac": round(pipe_gbs / PEAK_HBM_GBS, 4)},
-        "phases_ms": {k: round(v[0], 3) for k, v in ph.items()},
+        "phases_ms_per_step": {k: round(v[0] / args.steps, 3) for k, v in ph.items()},
     }
     if not args.no_cpu_baseline:
         del frames[2:]

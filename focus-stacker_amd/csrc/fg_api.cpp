@@ -229,7 +229,7 @@ struct PhaseMark {
     int phase;
     hipEvent_t a;
     PhaseMark(fg_stack* st, int ph) : s(st), phase(ph), a(nullptr) {
-        if (s->profiling && (a = take_event(s))) (void)hipEventRecord(a, s->stream);
+        if (ph >= 0 && s->profiling && (a = take_event(s))) (void)hipEventRecord(a, s->stream);
     }
     ~PhaseMark() {
         if (!a) return;
@@ -409,7 +409,16 @@ fg::Pass2Args pass2_args(fg_stack* s, int k0, int k1, int slot_base) {
 
 int run_pass2(fg_stack* s, int k0, int k1, int slot_base) {
     PhaseMark pm(s, FG_PHASE_PASS2);
-    HIP_TRY(fg::launch_pass2(pass2_args(s, k0, k1, slot_base), s->stream), "pass2 kernels");
+    fg::Pass2Args a = pass2_args(s, k0, k1, slot_base);
+    {
+        PhaseMark pm0(s, FG_PHASE_PASS2_LEVEL0);
+        a.level_begin = 0;
+        a.level_end = 1;
+        HIP_TRY(fg::launch_pass2(a, s->stream), "pass2 level-0 kernel");
+    }
+    a.level_begin = 1;
+    a.level_end = 0;
+    HIP_TRY(fg::launch_pass2(a, s->stream), "pass2 kernels");
     return FG_OK;
 }
 
@@ -608,7 +617,7 @@ static int add_frame(fg_stack* s, const fg_image_desc* d, const void* px, bool d
     if (rc) return rc;
     const uint32_t k = s->added;
     {
-        PhaseMark pm(s, FG_PHASE_INGEST);
+        PhaseMark pm(s, dev ? -1 : FG_PHASE_INGEST);   // device frames enqueue no copy
         rc = describe_frame(s, d, px, dev, pose, roi, &s->fdesc[k]);
     }
     if (rc) return rc;
@@ -857,12 +866,15 @@ int fg_stack_reset(fg_stack* s) {
     s->stage_used = 0;
     s->finished = false;
     s->have_src = false;
-    recycle_events(s);
     return FG_OK;
 }
 
 int fg_stack_set_profiling(fg_stack* s, int enable) {
     if (!s) return fail(FG_ERR_INVALID_ARG, "null stack");
+    if (enable) {
+        HIP_TRY(hipStreamSynchronize(s->stream), "sync");
+        recycle_events(s);
+    }
     s->profiling = enable != 0;
     return FG_OK;
 }

@@ -98,6 +98,7 @@ struct Pass2Args {
     float* acc;
     float* contrib;           // scratch for the ordered sum of small levels
     size_t contrib_floats;
+    int level_begin, level_end;   // levels [level_begin, level_end) of the table (end 0 = all)
 };
 hipError_t launch_pass2(const Pass2Args& a, hipStream_t s);
 size_t pass2_contrib_floats(const LevelTable* lt, uint32_t frames);

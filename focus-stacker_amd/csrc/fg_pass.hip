@@ -653,7 +653,8 @@ static hipError_t pass2_t(const Pass2Args& a, hipStream_t s) {
     const int nk = a.k1 - a.k0;
     if (nk <= 0) return hipSuccess;
     const size_t base1 = L > 1 ? lt.lv[1].off : 0;
-    for (int l = 0; l < L; l++) {
+    const int lend = a.level_end > 0 && a.level_end < L ? a.level_end : L;
+    for (int l = a.level_begin; l < lend; l++) {
         LevelArgs g;
         g.w = (int)lt.lv[l].w;
         g.h = (int)lt.lv[l].h;

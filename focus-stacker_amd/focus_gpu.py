@@ -21,6 +21,8 @@ ERRORS = {
 SAMPLE_U8, SAMPLE_U16 = 0, 1
 METHOD_PYRAMID, METHOD_HYBRID = 2, 3
 DEBUG = dict(norm_sum=0, union_support=1, weights=2, acc_level=3, mask_level=4, gauss_level=5, soft=6, hard=7)
+# enum fg_phase order; pass2_level0 is the level-0 launch inside pass2
+PHASES = ("ingest", "pass1", "pass2", "finish", "pass2_level0")
 
 POSE_FIELDS = (
     "yaw", "pitch", "roll", "hfov_delta", "trans_x", "trans_y", "trans_z",
@@ -259,10 +261,10 @@ class Stack:
         check(lib().fg_stack_set_profiling(self.h_, 1 if on else 0))
 
     def phase_times(self) -> dict:
-        ms = np.zeros(4, np.float64)
-        n = np.zeros(4, np.uint32)
+        ms = np.zeros(len(PHASES), np.float64)
+        n = np.zeros(len(PHASES), np.uint32)
         check(lib().fg_stack_phase_times(self.h_, _p(ms), _p(n)))
-        names = ("ingest", "pass1", "pass2", "finish")
+        names = PHASES
         return {k: (float(ms[i]), int(n[i])) for i, k in enumerate(names)}
 
     def stream(self) -> int:

@@ -167,8 +167,17 @@ int fg_stack_sync(fg_stack* stack);
 int fg_stack_reset(fg_stack* stack);
 
 /* Device time per phase, from HIP events recorded on the stack's stream while profiling is
- * on (accumulated since the last reset).  ms[k] / launches[k] for k < FG_PHASE_COUNT. */
-enum fg_phase { FG_PHASE_INGEST = 0, FG_PHASE_PASS1 = 1, FG_PHASE_PASS2 = 2, FG_PHASE_FINISH = 3, FG_PHASE_COUNT = 4 };
+ * on, accumulated over every stack run since fg_stack_set_profiling(stack, 1).  ms[k] is the
+ * total and launches[k] the number of bracketed regions, for k < FG_PHASE_COUNT:
+ *   INGEST        host-frame uploads;
+ *   PASS1         the pass-1 launch (remap + weights + norm sum), one per flush;
+ *   PASS2         every pass-2 launch of a stack (all levels);
+ *   FINISH        collapse, quantize and the hybrid merge;
+ *   PASS2_LEVEL0  the level-0 pass-2 launch alone (inside PASS2). */
+enum fg_phase {
+    FG_PHASE_INGEST = 0, FG_PHASE_PASS1 = 1, FG_PHASE_PASS2 = 2, FG_PHASE_FINISH = 3, FG_PHASE_PASS2_LEVEL0 = 4,
+    FG_PHASE_COUNT = 5
+};
 int fg_stack_set_profiling(fg_stack* stack, int enable);
 int fg_stack_phase_times(fg_stack* stack, double* ms, uint32_t* launches);
 
