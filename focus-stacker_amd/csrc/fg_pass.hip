@@ -67,62 +67,209 @@ __device__ __forceinline__ void frame_rgba(const FrameDesc& f, int x, int y, uin
     }
 }
 
+// frame_rgba out of line, for the rare pixels the fast remap path hands back
+template <typename T>
+__device__ __attribute__((noinline)) uint64_t frame_rgba_exact(const FrameDesc& f, int x, int y) {
+    uint32_t o[4];
+    frame_rgba<T>(f, x, y, o);
+    return (uint64_t)o[0] | ((uint64_t)o[1] << 16) | ((uint64_t)o[2] << 32) | ((uint64_t)o[3] << 48);
+}
+
 // ---------------------------------------------------------------------------
 // pass 1
 // ---------------------------------------------------------------------------
+// A block owns a region of 64 columns x RH rows of the ROI: an output tile of (64 - 2b) x
+// (RH - 2b) pixels plus the contrast halo b = window / 2.  Wave w remaps region rows w, w+4, ...
+// (lane = column), so every LDS row is one wave and every global store is one contiguous run.
 
-constexpr int P1W = 64, P1H = 16;
+constexpr int P1C = 64;
 
-size_t pass1_lds_bytes(int window, int sb) {
-    const int b = window / 2;
-    const size_t cols = P1W + 2 * b, rows = P1H + 2 * b;
-    return P1H * cols * (8 + 8 + 4) + rows * cols * ((sb == 1 ? 4 : 8) + 4);
+int pass1_rows(int window) { return window / 2 <= 4 ? 32 : 64; }
+
+// dynamic LDS of k_pass1 (byte offsets of every array in Pass1Lds): region gray and alpha (and
+// colour for the hybrid method), window-column sums, the tile's running norm sum and union
+// support and, for the hybrid method, its winning weight and colour
+struct Pass1Lds {
+    size_t gray, alpha, colour, csum, csq, ccnt, tS, tU, tB, tH, total;
+};
+__host__ __device__ inline Pass1Lds pass1_lds(int rh, int th, int sb, bool hybrid) {
+    Pass1Lds o;
+    const size_t reg = (size_t)rh * P1C, til = (size_t)th * P1C, pt = sb == 1 ? 4 : 8;
+    size_t at = 0;
+    o.csum = at;   at += til * 8;
+    o.csq = at;    at += til * 8;
+    o.gray = at;   at += reg * 4;
+    o.tS = at;     at += til * 4;
+    o.tB = at;     at += hybrid ? til * 4 : 0;
+    o.colour = at; at += hybrid ? reg * pt : 0;
+    o.tH = at;     at += hybrid ? til * pt : 0;
+    o.alpha = at;  at += reg * 2;
+    o.ccnt = at;   at += til;
+    o.tU = at;     at += til;
+    o.total = (at + 15) & ~(size_t)15;
+    return o;
 }
 
+size_t pass1_lds_bytes(int window, int sb, bool hybrid) {
+    const int b = window / 2, rh = pass1_rows(window);
+    return pass1_lds(rh, rh - 2 * b, sb, hybrid).total;
+}
+
+// 1 / d to within a few ulp: hardware estimate and two Newton steps.
+__device__ __forceinline__ double rcp_f64(double d) {
+    double r = __builtin_amdgcn_rcp(d);
+    double e = fma(-d, r, 1.0);
+    r = fma(r, e, r);
+    e = fma(-d, r, 1.0);
+    return fma(r, e, r);
+}
+
+// a / b correctly rounded from y = RN(1/b) by one Markstein correction step (P. Markstein,
+// "Computation of elementary functions on the IBM RISC System/6000 processor", IBM J. R&D 34,
+// 1990): q = a*y, r = a - q*b (exact with an fma), RN(q + r*y).
+__device__ __forceinline__ double div_markstein(double a, double b, double y) {
+    const double q = a * y;
+    const double r = fma(-q, b, a);
+    return fma(r, y, q);
+}
+__device__ __forceinline__ float div_markstein(float a, float b, float y) {
+    const float q = a * y;
+    const float r = fmaf(-q, b, a);
+    return fmaf(r, y, q);
+}
+
+// LDS-only workgroup barrier: orders LDS accesses across the block without waiting for the
+// block's outstanding global stores (no thread reads another thread's global stores in a launch).
+__device__ __forceinline__ void lds_barrier() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+// One pixel of the homography fast path between its gather and its arithmetic: the bilinear
+// weights and the 12 source samples (rows y0, y0+1 x columns x0, x0+1 x rgb).
+struct FastTaps {
+    double w00, w10, w01, w11;
+    uint32_t t[12];
+    bool ok;  // interior of the source: the taps are valid
+};
+
+// Source coordinate of region pixel (gx, gy) by the homography of map_pixel with one reciprocal
+// in place of its two divisions, and the gather of the interior case of samplePixelBilinear
+// (src/remap.zig:788-839, 886-937), 0 <= x < w-1 and 0 <= y < h-1.  Pixels outside that case
+// leave ok = false for the exact path.
 template <typename T>
+__device__ __forceinline__ void gather_fast(const RemapParams& p, const T* __restrict__ src, uint32_t sw, double wlim,
+                                            double hlim, double cx, double cy, double cz, int gy, FastTaps& g) {
+    g.ok = false;
+    const double pyd = (double)(p.roi_top + gy) - p.dcy;
+    const double lx = (cx + p.B[1] * pyd) + p.B[2] * p.neg_d;
+    const double ly = (cy + p.B[4] * pyd) + p.B[5] * p.neg_d;
+    const double lz = (cz + p.B[7] * pyd) + p.B[8] * p.neg_d;
+    const double denom = -lz;
+    if (!(p.legacy ? denom > 1e-6 : fabs(denom) > 1e-6)) return;
+    const double inv = rcp_f64(denom);
+    const double x = p.scx + (p.focal * (lx * inv) + p.csx);
+    const double y = p.scy + (p.focal * (ly * inv) + p.csy);
+    if (!(x >= 0.0 && y >= 0.0 && x < wlim && y < hlim)) return;
+    const __attribute__((address_space(1))) T* px = as_global(src);
+    const double fx0 = floor(x), fy0 = floor(y);
+    const uint32_t x0 = (uint32_t)(int)fx0, y0 = (uint32_t)(int)fy0;
+    const double fx = x - fx0, fy = y - fy0;
+    const double omx = 1.0 - fx, omy = 1.0 - fy;
+    g.w00 = omx * omy;
+    g.w10 = fx * omy;
+    g.w01 = omx * fy;
+    g.w11 = fx * fy;
+    const uint32_t rs = sw * 3u;
+    const uint32_t b00 = y0 * rs + x0 * 3u;
+#pragma unroll
+    for (int c = 0; c < 6; c++) {
+        g.t[c] = px[b00 + c];
+        g.t[6 + c] = px[b00 + rs + c];
+    }
+    g.ok = true;
+}
+
+// The arithmetic of the interior case on gathered taps: the reference's f64 weighted sums, the
+// quotient by the weight sum through a reciprocal.  Returns false if a channel lies within
+// `band` of a rounding tie (the caller then redoes the pixel exactly).
+__device__ __forceinline__ bool resolve_fast(const FastTaps& g, double band, uint32_t o[3]) {
+    const double sup = ((g.w00 + g.w10) + g.w01) + g.w11;
+    const double inv = rcp_f64(sup);
+    bool ok = true;
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        const double v = (((double)g.t[c] * g.w00 + (double)g.t[3 + c] * g.w10) + (double)g.t[6 + c] * g.w01) +
+                         (double)g.t[9 + c] * g.w11;
+        const double t = v * inv + 0.5;
+        const double fl = floor(t);
+        const double d = t - fl;
+        ok = ok && d > band && d < 1.0 - band;
+        o[c] = (uint32_t)fl;
+    }
+    return ok;
+}
+
+template <typename T, int RH>
 __global__ void __launch_bounds__(256) k_pass1(Pass1Args a) {
     using P = Pack<T>;
     using PT = typename P::type;
     extern __shared__ __align__(16) unsigned char smem[];
     const int b = (int)a.window / 2;
-    const int cols = P1W + 2 * b, rows = P1H + 2 * b;
-    const int n_reg = rows * cols, n_col = P1H * cols;
-    double* csum = (double*)smem;
-    double* csq = csum + n_col;
-    PT* px = (PT*)(csq + n_col);
-    float* gray = (float*)(px + n_reg);
-    int* ccnt = (int*)(gray + n_reg);
-
-    const int W = (int)a.w, H = (int)a.h;
-    const int X0 = blockIdx.x * P1W, Y0 = blockIdx.y * P1H;
-    const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
-    const bool contrast_ok = W >= (int)a.window && H >= (int)a.window;
-    const size_t fpx = (size_t)W * H;
+    const int th = RH - 2 * b, tw = P1C - 2 * b;
+    const int rpw = (th + 3) / 4;
     const bool acc = a.norm_sum != nullptr;
     const bool hyb = a.best != nullptr;
+    const Pass1Lds L = pass1_lds(RH, th, (int)sizeof(T), hyb);
+    float* gray = (float*)(smem + L.gray);            // [RH][64] gray, -1 where alpha == 0 or outside
+    uint16_t* alph = (uint16_t*)(smem + L.alpha);     // [RH][64] alpha
+    PT* colour = (PT*)(smem + L.colour);              // [RH][64] hybrid: packed rgb
+    double* csum = (double*)(smem + L.csum);          // [th][64] window-column sums of gray
+    double* csq = (double*)(smem + L.csq);            // [th][64] ... of the f32 square
+    uint8_t* ccnt = smem + L.ccnt;                    // [th][64] ... of supported samples
+    float* tS = (float*)(smem + L.tS);                // [th][64] norm sum of the tile
+    uint8_t* tU = smem + L.tU;                        // [th][64] union support
+    float* tB = (float*)(smem + L.tB);                // [th][64] hybrid: best weight
+    PT* tH = (PT*)(smem + L.tH);                      // [th][64] hybrid: its colour
 
-    float s_acc[4], b_acc[4];
-    uint32_t u_acc[4], h_acc[4][3];
-#pragma unroll
-    for (int j = 0; j < 4; j++) {
-        const int x = X0 + tx, y = Y0 + ty + 4 * j;
-        s_acc[j] = 0.0f;
-        u_acc[j] = 0;
-        b_acc[j] = 0.0f;
-        h_acc[j][0] = h_acc[j][1] = h_acc[j][2] = 0;
-        if (x < W && y < H) {
-            const size_t i = (size_t)y * W + x;
+    const int W = (int)a.w, H = (int)a.h;
+    const int X0 = blockIdx.x * tw, Y0 = blockIdx.y * th;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const bool contrast_ok = W >= (int)a.window && H >= (int)a.window;
+    const size_t fpx = (size_t)W * H;
+    const int r0 = wave * rpw;                        // this wave's output rows [r0, r0 + rpw) of the tile
+    const int r1 = r0 + rpw < th ? r0 + rpw : th;
+    const int ox = X0 + lane - b;                     // output column of this lane
+    const bool col_ok = lane >= b && lane < P1C - b && ox < W;
+    const int nfull = (int)(a.window * a.window);
+    const double nf_full = (double)nfull, nf_full1 = (double)(nfull - 1);
+    const double r_full = 1.0 / nf_full, r_full1 = 1.0 / nf_full1;
+    const double band = sizeof(T) == 1 ? 1e-8 : 4e-6;  // covers the reciprocal coordinates' error
+
+#pragma unroll 1
+    for (int r = r0; r < r1; r++) {
+        const int y = Y0 + r;
+        float sv = 0.0f, bv = 0.0f;
+        uint32_t uv = 0;
+        PT hv = P::make(0, 0, 0, 0);
+        if (col_ok && y < H) {
+            const size_t i = (size_t)y * W + ox;
             if (acc) {
-                s_acc[j] = a.norm_sum[i];
-                u_acc[j] = a.union_sup[i];
+                sv = a.norm_sum[i];
+                uv = a.union_sup[i];
             }
             if (hyb) {
                 const T* hd = (const T*)a.hard + i * 3;
-                b_acc[j] = a.best[i];
-                h_acc[j][0] = hd[0];
-                h_acc[j][1] = hd[1];
-                h_acc[j][2] = hd[2];
+                bv = a.best[i];
+                hv = P::make(hd[0], hd[1], hd[2], 0);
             }
+        }
+        tS[r * P1C + lane] = sv;
+        tU[r * P1C + lane] = (uint8_t)uv;
+        if (hyb) {
+            tB[r * P1C + lane] = bv;
+            tH[r * P1C + lane] = hv;
         }
     }
 
@@ -131,124 +278,191 @@ __global__ void __launch_bounds__(256) k_pass1(Pass1Args a) {
         const size_t slot = (size_t)(k - a.slot_base);
         T* rgba = (T*)a.rgba + slot * fpx * 4;
         float* wout = a.weights + slot * fpx;
-        // remap the tile and its contrast halo; gray < 0 marks unsupported or outside
-        for (int idx = threadIdx.x; idx < n_reg; idx += 256) {
-            const int gx = X0 - b + idx % cols, gy = Y0 - b + idx / cols;
-            uint32_t o[4] = {0, 0, 0, 0};
-            float g = -1.0f;
-            if (gx >= 0 && gy >= 0 && gx < W && gy < H) {
-                frame_rgba<T>(f, gx, gy, o);
-                if (o[3] > 0) g = (float)(o[0] + o[1] + o[2]) / SampleMax<T>::gray3;
-            }
-            px[idx] = P::make(o[0], o[1], o[2], o[3]);
-            gray[idx] = g;
+        // remap the region (remapRigidImage); gray (fillAverageFromLoaded) where alpha > 0
+        const int gx = X0 - b + lane;
+        const bool fast = f.kind == 0 && f.p.mode == REMAP_HOMOGRAPHY;
+        const RemapParams& p = f.p;
+        double cx = 0.0, cy = 0.0, cz = 0.0;
+        if (fast) {
+            const double pxd = (double)(p.roi_left + gx) - p.dcx;
+            cx = p.B[0] * pxd;
+            cy = p.B[3] * pxd;
+            cz = p.B[6] * pxd;
         }
-        __syncthreads();
-        // column sums of the window rows: f64 sums of x and of the f32 square x*x
-        // (computeRange src/fuse/contrast.zig:201-217)
-        if (contrast_ok) {
-            for (int idx = threadIdx.x; idx < n_col; idx += 256) {
-                const int r = idx / cols, c = idx % cols;
-                double s = 0.0, q = 0.0;
-                int n = 0;
-                for (int dy = 0; dy <= 2 * b; dy++) {
-                    const float v = gray[(r + dy) * cols + c];
-                    if (v >= 0.0f) {
-                        s += (double)v;
-                        q += (double)(v * v);
-                        n++;
+        const double wlim = (double)f.sw - 1.0, hlim = (double)f.sh - 1.0;
+        // two region rows per step (r, r + 4): both gathers are in flight before either's arithmetic
+#pragma unroll 1
+        for (int r2 = wave; r2 < RH; r2 += 8) {
+            FastTaps gt[2];
+            bool in[2];
+#pragma unroll
+            for (int h2 = 0; h2 < 2; h2++) {
+                const int gy = Y0 - b + r2 + 4 * h2;
+                in[h2] = gx >= 0 && gy >= 0 && gx < W && gy < H;
+                gt[h2].ok = false;
+                if (fast && in[h2]) gather_fast<T>(p, (const T*)f.src, f.sw, wlim, hlim, cx, cy, cz, gy, gt[h2]);
+            }
+#pragma unroll
+            for (int h2 = 0; h2 < 2; h2++) {
+                const int r = r2 + 4 * h2;
+                const int gy = Y0 - b + r;
+                uint32_t o[4] = {0, 0, 0, 0};
+                if (in[h2]) {
+                    if (gt[h2].ok && resolve_fast(gt[h2], band, o)) {
+                        o[3] = SampleMax<T>::i;
+                    } else {
+                        const uint64_t e = frame_rgba_exact<T>(f, gx, gy);
+                        o[0] = (uint32_t)e & 0xffffu;
+                        o[1] = (uint32_t)(e >> 16) & 0xffffu;
+                        o[2] = (uint32_t)(e >> 32) & 0xffffu;
+                        o[3] = (uint32_t)(e >> 48);
                     }
                 }
-                csum[idx] = s;
-                csq[idx] = q;
-                ccnt[idx] = n;
+                float g = -1.0f;
+                if (o[3] > 0) {
+                    const float sc = (float)((o[0] + o[1]) + o[2]);
+                    g = div_markstein(sc, SampleMax<T>::gray3, 1.0f / SampleMax<T>::gray3);
+                }
+                const PT c = P::make(o[0], o[1], o[2], o[3]);
+                if (r >= b && r < RH - b && lane >= b && lane < P1C - b && gx < W && gy < H) {
+                    const size_t i = (size_t)gy * W + gx;  // an output pixel of the tile
+                    if (sizeof(T) == 1)
+                        ((uint32_t*)rgba)[i] = *(const uint32_t*)&c;
+                    else
+                        ((uint2*)rgba)[i] = *(const uint2*)&c;
+                }
+                gray[r * P1C + lane] = g;
+                alph[r * P1C + lane] = (uint16_t)o[3];
+                if (hyb) colour[r * P1C + lane] = c;
             }
         }
-        __syncthreads();
-#pragma unroll
-        for (int j = 0; j < 4; j++) {
-            const int r = ty + 4 * j;
-            const int x = X0 + tx, y = Y0 + r;
-            if (x >= W || y >= H) continue;
-            const size_t i = (size_t)y * W + x;
-            const PT c = px[(r + b) * cols + tx + b];
-            if (sizeof(T) == 1)
-                ((uint32_t*)rgba)[i] = *(const uint32_t*)&c;
-            else
-                ((uint2*)rgba)[i] = *(const uint2*)&c;
+        lds_barrier();
+        // window-column sums of this wave's output rows, sliding down the column
+        // (computeRange src/fuse/contrast.zig:201-217: f64 sums of x and of the f32 product x*x)
+        if (contrast_ok && r0 < r1) {
+            double s = 0.0, q = 0.0;
+            int n = 0;
+#pragma unroll 1
+            for (int dy = 0; dy <= 2 * b; dy++) {
+                const float v = gray[(r0 + dy) * P1C + lane];
+                if (v >= 0.0f) {
+                    s += (double)v;
+                    q += (double)(v * v);
+                    n++;
+                }
+            }
+            csum[r0 * P1C + lane] = s;
+            csq[r0 * P1C + lane] = q;
+            ccnt[r0 * P1C + lane] = n;
+#pragma unroll 1
+            for (int r = r0 + 1; r < r1; r++) {
+                const float vi = gray[(r + 2 * b) * P1C + lane];
+                const float vo = gray[(r - 1) * P1C + lane];
+                if (vi >= 0.0f) {
+                    s += (double)vi;
+                    q += (double)(vi * vi);
+                    n++;
+                }
+                if (vo >= 0.0f) {
+                    s -= (double)vo;
+                    q -= (double)(vo * vo);
+                    n--;
+                }
+                csum[r * P1C + lane] = s;
+                csq[r * P1C + lane] = q;
+                ccnt[r * P1C + lane] = n;
+            }
+        }
+        lds_barrier();
+#pragma unroll 1
+        for (int r = r0; r < r1; r++) {
+            const int y = Y0 + r;
+            if (!col_ok || y >= H) continue;
+            const size_t i = (size_t)y * W + ox;
             float stdw = 0.0f;
-            if (contrast_ok && x >= b && x < W - b && y >= b && y < H - b && gray[(r + b) * cols + tx + b] >= 0.0f) {
+            if (contrast_ok && ox >= b && ox < W - b && y >= b && y < H - b && gray[(r + b) * P1C + lane] >= 0.0f) {
                 double s = 0.0, q = 0.0;
                 int n = 0;
                 for (int dx = 0; dx <= 2 * b; dx++) {
-                    const int ci = r * cols + tx + dx;
+                    const int ci = r * P1C + lane - b + dx;
                     s += csum[ci];
                     q += csq[ci];
                     n += ccnt[ci];
                 }
                 if (n > 1) {  // sampleStdDev src/fuse/contrast.zig:284-290
-                    const double denom = (double)(n - 1), nf = (double)n;
-                    double var = (q - ((s * s) / nf)) / denom;
+                    double m, var;
+                    if (n == nfull) {
+                        m = div_markstein(s * s, nf_full, r_full);
+                        var = div_markstein(q - m, nf_full1, r_full1);
+                    } else {
+                        const double nf = (double)n, den = (double)(n - 1);
+                        m = (s * s) / nf;
+                        var = (q - m) / den;
+                    }
                     if (!(var > 0.0)) var = 0.0;
                     stdw = (float)sqrt(var);
                 }
             }
-            const uint32_t alpha = P::ch(c, 3);
-            const float wv = stdw * ((float)alpha / SampleMax<T>::f);  // applySupportInto
+            const uint32_t alpha = alph[(r + b) * P1C + lane];
+            const float af = alpha == SampleMax<T>::i
+                                 ? 1.0f
+                                 : div_markstein((float)alpha, SampleMax<T>::f, 1.0f / SampleMax<T>::f);
+            const float wv = stdw * af;  // applySupportInto
             wout[i] = wv;
+            const int t = r * P1C + lane;
             if (acc) {
-                s_acc[j] = s_acc[j] + wv;
-                if (alpha > 0) u_acc[j] = 1;
+                tS[t] = tS[t] + wv;
+                if (alpha > 0) tU[t] = 1;
             }
-            if (hyb && wv > b_acc[j]) {  // updateRange: strictly greater wins, first frame on ties
-                b_acc[j] = wv;
-                h_acc[j][0] = P::ch(c, 0);
-                h_acc[j][1] = P::ch(c, 1);
-                h_acc[j][2] = P::ch(c, 2);
+            if (hyb && wv > tB[t]) {  // updateRange: strictly greater wins, first frame on ties
+                tB[t] = wv;
+                tH[t] = colour[(r + b) * P1C + lane];
             }
         }
-        __syncthreads();
+        lds_barrier();
     }
 
-#pragma unroll
-    for (int j = 0; j < 4; j++) {
-        const int x = X0 + tx, y = Y0 + ty + 4 * j;
-        if (x >= W || y >= H) continue;
-        const size_t i = (size_t)y * W + x;
+#pragma unroll 1
+    for (int r = r0; r < r1; r++) {
+        const int y = Y0 + r;
+        if (!col_ok || y >= H) continue;
+        const size_t i = (size_t)y * W + ox;
+        const int t = r * P1C + lane;
         if (acc) {
-            a.norm_sum[i] = s_acc[j];
-            a.union_sup[i] = (uint8_t)u_acc[j];
+            a.norm_sum[i] = tS[t];
+            a.union_sup[i] = (uint8_t)tU[t];
         }
         if (hyb) {
             T* hd = (T*)a.hard + i * 3;
-            a.best[i] = b_acc[j];
-            hd[0] = (T)h_acc[j][0];
-            hd[1] = (T)h_acc[j][1];
-            hd[2] = (T)h_acc[j][2];
+            const PT h = tH[t];
+            a.best[i] = tB[t];
+            hd[0] = (T)P::ch(h, 0);
+            hd[1] = (T)P::ch(h, 1);
+            hd[2] = (T)P::ch(h, 2);
         }
     }
 }
 
+template <typename T, int RH>
+static hipError_t launch_pass1_t(const Pass1Args& a, hipStream_t s) {
+    const int b = (int)a.window / 2, th = RH - 2 * b, tw = P1C - 2 * b;
+    const size_t lds = pass1_lds_bytes((int)a.window, (int)sizeof(T), a.best != nullptr);
+    if (lds > 65536) {
+        hipError_t e =
+            hipFuncSetAttribute((const void*)k_pass1<T, RH>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess) return e;
+    }
+    dim3 grid((a.w + tw - 1) / tw, (a.h + th - 1) / th);
+    hipLaunchKernelGGL((k_pass1<T, RH>), grid, dim3(256), lds, s, a);
+    return hipGetLastError();
+}
+
 hipError_t launch_pass1(const Pass1Args& a, hipStream_t s) {
     if (a.k1 <= a.k0) return hipSuccess;
-    const size_t lds = pass1_lds_bytes((int)a.window, a.sample_bytes);
-    dim3 grid((a.w + P1W - 1) / P1W, (a.h + P1H - 1) / P1H);
-    if (a.sample_bytes == 1) {
-        if (lds > 65536) {
-            hipError_t e = hipFuncSetAttribute((const void*)k_pass1<uint8_t>,
-                                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-            if (e != hipSuccess) return e;
-        }
-        hipLaunchKernelGGL(k_pass1<uint8_t>, grid, dim3(256), lds, s, a);
-    } else {
-        if (lds > 65536) {
-            hipError_t e = hipFuncSetAttribute((const void*)k_pass1<uint16_t>,
-                                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-            if (e != hipSuccess) return e;
-        }
-        hipLaunchKernelGGL(k_pass1<uint16_t>, grid, dim3(256), lds, s, a);
-    }
-    return hipGetLastError();
+    if (a.window < 3 || a.window > 31 || (a.window & 1) == 0) return hipErrorInvalidValue;
+    const bool big = pass1_rows((int)a.window) == 64;
+    if (a.sample_bytes == 1) return big ? launch_pass1_t<uint8_t, 64>(a, s) : launch_pass1_t<uint8_t, 32>(a, s);
+    return big ? launch_pass1_t<uint16_t, 64>(a, s) : launch_pass1_t<uint16_t, 32>(a, s);
 }
 
 // ---------------------------------------------------------------------------
@@ -432,7 +646,7 @@ __global__ void __launch_bounds__(256) k_pass2(Pass2Args a, LevelArgs g) {
                 rM[idx] = v.w;
             }
         }
-        __syncthreads();
+        lds_barrier();
         if (!g.top) {
             // B: reduceRgbWithSupport / reduceScalarWithSupport (src/fuse/pyramid.zig:997-1099),
             // two vertically adjacent outputs per thread.  A tap without support adds
@@ -481,7 +695,7 @@ __global__ void __launch_bounds__(256) k_pass2(Pass2Args a, LevelArgs g) {
                     nB[o] = e2;
                 }
             }
-            __syncthreads();
+            lds_barrier();
             // C: horizontal pass of EXPAND for every level-(l+1) row of the region and tile column
             // (horizontalExpandRgbRow :1748-1775 above 512 px; the running sums of
             // expandRgbExactNoWrap :1197-1214 at or below 512 px, zero outside the level)
@@ -530,7 +744,7 @@ __global__ void __launch_bounds__(256) k_pass2(Pass2Args a, LevelArgs g) {
                 hG[idx] = h1;
                 hB[idx] = h2;
             }
-            __syncthreads();
+            lds_barrier();
         }
         // D: vertical pass of EXPAND, laplacian, accumulate
         // acc[l] += (G[l] - EXPAND(G[l+1])) * M[l]  (accumulateLapRowsSequential :1506-1531),
@@ -586,7 +800,7 @@ __global__ void __launch_bounds__(256) k_pass2(Pass2Args a, LevelArgs g) {
                 ab[j] = ab[j] + lap.z * m;
             }
         }
-        __syncthreads();
+        lds_barrier();
     }
 
     if (!SPLIT) {
