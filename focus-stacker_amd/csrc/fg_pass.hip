@@ -565,18 +565,63 @@ __device__ __forceinline__ void reduce_row(const float* rR, const float* rG, con
     }
 }
 
+// One tap of the reduce on a fully supported window: sum + value * W as the reference rounds it.
+// For power-of-two W the product is exact, so the fused form rounds once, exactly like
+// RN(sum + RN(value * W)); the other weights (6, 24, 36) keep the separate product.
+template <int W>
+__device__ __forceinline__ float red_tap(float sum, float v) {
+    if constexpr ((W & (W - 1)) == 0)
+        return fmaf(v, (float)W, sum);
+    else
+        return sum + v * (float)W;
+}
+
+// Adds one LDS row (columns base..base+4, kx order) with vertical weight KY to the four sums
+// (r, g, b, mask) of a window whose 25 taps all have image and mask support.
+template <int KY>
+__device__ __forceinline__ void reduce_row_full(const float* rR, const float* rG, const float* rB, const float* rM,
+                                                int base, float s[4]) {
+    const float* ch[4] = {rR, rG, rB, rM};
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+        const float2 u0 = *(const float2*)&ch[c][base], u1 = *(const float2*)&ch[c][base + 2];
+        const float u4 = ch[c][base + 4];
+        float a = s[c];
+        a = red_tap<KY * 1>(a, u0.x);
+        a = red_tap<KY * 4>(a, u0.y);
+        a = red_tap<KY * 6>(a, u1.x);
+        a = red_tap<KY * 4>(a, u1.y);
+        a = red_tap<KY * 1>(a, u4);
+        s[c] = a;
+    }
+}
+
+// w / s correctly rounded from y = RN(1/s) (Markstein correction); the quotient of a tiny
+// numerator, where the correction's residual could leave the normal range, is divided directly.
+__device__ __forceinline__ float div_by_recip(float w, float s, float y) {
+    if (w != 0.0f && fabsf(w) < 1e-30f) return w / s;
+    const float q = w * y;
+    const float r = fmaf(-q, s, w);
+    return fmaf(r, y, q);
+}
+
 template <typename T, bool L0, int TW, int TH, bool SPLIT>
 __global__ void __launch_bounds__(256) k_pass2(Pass2Args a, LevelArgs g) {
     using TL = Tile<TW, TH>;
+    using PT = typename Pack<T>::type;
     constexpr int RW = TL::RW, RH = TL::RH, NW = TL::NW, NH = TL::NH;
+    constexpr int NREG = RH * RW;
+    constexpr int NPF = (NREG + 255) / 256;  // region pixels per thread
     // level-l region, planar: red (support flag in the sign), green, blue, mask (flag in the sign)
-    __shared__ __align__(16) float rR[RH * RW];
-    __shared__ __align__(16) float rG[RH * RW];
-    __shared__ __align__(16) float rB[RH * RW];
-    __shared__ __align__(16) float rM[RH * RW];
-    __shared__ float sS[L0 ? RH * RW : 1];        // norm sum, -1 where no frame has support
+    __shared__ __align__(16) float rR[NREG];
+    __shared__ __align__(16) float rG[NREG];
+    __shared__ __align__(16) float rB[NREG];
+    __shared__ __align__(16) float rM[NREG];
+    __shared__ float sS[L0 ? NREG : 1];        // norm sum, -1 where no frame has support
+    __shared__ float sY[L0 ? NREG : 1];        // RN(1 / norm sum)
     __shared__ float nR[NH * NW], nG[NH * NW], nB[NH * NW];   // level-(l+1) gaussian (values)
     __shared__ float hR[NH * TW], hG[NH * TW], hB[NH * TW];   // its horizontal expansion per tile column
+    __shared__ int sPart[2];                   // per frame parity: some region pixel lacks support
 
     const int X0 = blockIdx.x * TW, Y0 = blockIdx.y * TH;
     const int tx = threadIdx.x % TW, ty = threadIdx.x / TW;
@@ -600,76 +645,130 @@ __global__ void __launch_bounds__(256) k_pass2(Pass2Args a, LevelArgs g) {
             ab[j] = accp[i + 2 * n];
         }
     }
+    if (threadIdx.x < 2) sPart[threadIdx.x] = 0;
     if (L0) {
 #pragma unroll 1
-        for (int idx = threadIdx.x; idx < RH * RW; idx += 256) {
+        for (int idx = threadIdx.x; idx < NREG; idx += 256) {
             const int gx = X0 - 4 + idx % RW, gy = Y0 - 4 + idx / RW;
-            float v = -1.0f;
+            float v = -1.0f, y = 0.0f;
             if (gx >= 0 && gy >= 0 && gx < g.w && gy < g.h) {
                 const int i = gy * g.w + gx;
-                if (a.union_sup[i]) v = a.norm_sum[i];
+                if (a.union_sup[i]) {
+                    v = a.norm_sum[i];
+                    y = 1.0f / v;
+                }
             }
             sS[idx] = v;
+            sY[idx] = y;
         }
     }
 
+    // the next frame's level-l region, in flight while the current frame is reduced and expanded
+    PT pf_px[L0 ? NPF : 1];
+    float pf_w[L0 ? NPF : 1];
+    float4 pf_v[L0 ? 1 : NPF];
+    auto prefetch = [&](int k) {
+        const size_t slot = (size_t)(k - a.slot_base);
+        const PT* rg = L0 ? (const PT*)a.rgba + slot * (size_t)n : nullptr;
+        const float* wts = L0 ? a.weights + slot * (size_t)n : nullptr;
+        const float4* lv = L0 ? nullptr : a.lev + slot * a.lev_stride + g.lev_off;
+#pragma unroll
+        for (int j = 0; j < NPF; j++) {
+            const int idx = threadIdx.x + 256 * j;
+            const int gx = X0 - 4 + idx % RW, gy = Y0 - 4 + idx / RW;
+            if (idx < NREG && gx >= 0 && gy >= 0 && gx < g.w && gy < g.h) {
+                const int i = gy * g.w + gx;
+                if (L0) {
+                    pf_px[j] = rg[i];
+                    pf_w[j] = wts[i];
+                } else {
+                    pf_v[j] = lv[i];
+                }
+            }
+        }
+    };
+    prefetch(kb);
+    lds_barrier();
+
     for (int k = kb; k < ke; k++) {
         const size_t slot = (size_t)(k - a.slot_base);
-        // A: level-l region of frame k
+        // A: level-l region of frame k from the prefetch registers
         {
-            const T* rg = L0 ? (const T*)a.rgba + slot * (size_t)n * 4 : nullptr;
-            const float* wts = L0 ? a.weights + slot * (size_t)n : nullptr;
-            const float4* lv = L0 ? nullptr : a.lev + slot * a.lev_stride + g.lev_off;
-#pragma unroll 2
-            for (int idx = threadIdx.x; idx < RH * RW; idx += 256) {
+            bool full = true;
+#pragma unroll
+            for (int j = 0; j < NPF; j++) {
+                const int idx = threadIdx.x + 256 * j;
+                if (idx >= NREG) continue;
                 const int gx = X0 - 4 + idx % RW, gy = Y0 - 4 + idx / RW;
                 float4 v = make_float4(-1.0f, 0.0f, 0.0f, -1.0f);
                 if (gx >= 0 && gy >= 0 && gx < g.w && gy < g.h) {
-                    const int i = gy * g.w + gx;
                     if (L0) {
                         // fillRgbBase + fillBinarySupport (src/fuse/pyramid.zig:502-503) and
                         // normalizeWeightsInto (:303-316) with the union support (:202)
-                        const T* q = rg + (size_t)i * 4;
-                        const float r = (float)q[0];
-                        v.x = q[3] > 0 ? r : -(r + 1.0f);
-                        v.y = (float)q[1];
-                        v.z = (float)q[2];
+                        const PT q = pf_px[j];
+                        const float r = (float)Pack<T>::ch(q, 0);
+                        v.x = Pack<T>::ch(q, 3) > 0 ? r : -(r + 1.0f);
+                        v.y = (float)Pack<T>::ch(q, 1);
+                        v.z = (float)Pack<T>::ch(q, 2);
                         const float sv = sS[idx];
-                        v.w = sv < 0.0f ? -1.0f : normalized_mask(wts[i], sv, m_unsup);
+                        v.w = sv < 0.0f ? -1.0f : (sv > 1e-12f ? div_by_recip(pf_w[j], sv, sY[idx]) : m_unsup);
                     } else {
-                        v = lv[i];
+                        v = pf_v[j];
                     }
                 }
+                full = full && v.x >= 0.0f && v.w >= 0.0f;
                 rR[idx] = v.x;
                 rG[idx] = v.y;
                 rB[idx] = v.z;
                 rM[idx] = v.w;
             }
+            if (!full) sPart[k & 1] = 1;
+            if (threadIdx.x == 0) sPart[(k + 1) & 1] = 0;
         }
         lds_barrier();
+        if (k + 1 < ke) prefetch(k + 1);
         if (!g.top) {
             // B: reduceRgbWithSupport / reduceScalarWithSupport (src/fuse/pyramid.zig:997-1099),
-            // two vertically adjacent outputs per thread.  A tap without support adds
-            // value * 0 (= +-0) and weight 0, which leaves every sum unchanged, exactly as the
-            // reference's `continue`.
+            // two vertically adjacent outputs per thread.
+            const bool full = sPart[k & 1] == 0;
             constexpr int NP = NW * (NH / 2);
             float4* lvn = a.lev + slot * a.lev_stride + g.nlev_off;
             for (int p = threadIdx.x; p < NP; p += 256) {
                 const int oi = p % NW, oj = 2 * (p / NW);
-                RedAcc ra, rb;
                 const int base0 = (2 * oj) * RW + 2 * oi;
-                // rows 0-1 feed the upper output only, rows 2-4 both, rows 5-6 the lower only
+                float a0, a1, a2, aw, am, amw, b0, b1, b2, bw, bm, bmw;
+                if (full) {
+                    // every tap in bounds and supported: weight sums are exactly 256
+                    float sa[4] = {0.0f, 0.0f, 0.0f, 0.0f}, sb[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+                    reduce_row_full<1>(rR, rG, rB, rM, base0, sa);
+                    reduce_row_full<4>(rR, rG, rB, rM, base0 + RW, sa);
+                    reduce_row_full<6>(rR, rG, rB, rM, base0 + 2 * RW, sa);
+                    reduce_row_full<1>(rR, rG, rB, rM, base0 + 2 * RW, sb);
+                    reduce_row_full<4>(rR, rG, rB, rM, base0 + 3 * RW, sa);
+                    reduce_row_full<4>(rR, rG, rB, rM, base0 + 3 * RW, sb);
+                    reduce_row_full<1>(rR, rG, rB, rM, base0 + 4 * RW, sa);
+                    reduce_row_full<6>(rR, rG, rB, rM, base0 + 4 * RW, sb);
+                    reduce_row_full<4>(rR, rG, rB, rM, base0 + 5 * RW, sb);
+                    reduce_row_full<1>(rR, rG, rB, rM, base0 + 6 * RW, sb);
+                    a0 = sa[0]; a1 = sa[1]; a2 = sa[2]; am = sa[3]; aw = amw = 256.0f;
+                    b0 = sb[0]; b1 = sb[1]; b2 = sb[2]; bm = sb[3]; bw = bmw = 256.0f;
+                } else {
+                    // A tap without support adds value * 0 (= +-0) and weight 0, which leaves every
+                    // sum unchanged, exactly as the reference's `continue`.
+                    RedAcc ra, rb;
+                    // rows 0-1 feed the upper output only, rows 2-4 both, rows 5-6 the lower only
 #pragma unroll 1
-                for (int rr = 0; rr < 2; rr++)
-                    reduce_row<true, false>(rR, rG, rB, rM, base0 + rr * RW, kw_of(rr), 0.0f, ra, rb);
+                    for (int rr = 0; rr < 2; rr++)
+                        reduce_row<true, false>(rR, rG, rB, rM, base0 + rr * RW, kw_of(rr), 0.0f, ra, rb);
 #pragma unroll 1
-                for (int rr = 2; rr < 5; rr++)
-                    reduce_row<true, true>(rR, rG, rB, rM, base0 + rr * RW, kw_of(rr), kw_of(rr - 2), ra, rb);
+                    for (int rr = 2; rr < 5; rr++)
+                        reduce_row<true, true>(rR, rG, rB, rM, base0 + rr * RW, kw_of(rr), kw_of(rr - 2), ra, rb);
 #pragma unroll 1
-                for (int rr = 5; rr < 7; rr++)
-                    reduce_row<false, true>(rR, rG, rB, rM, base0 + rr * RW, 0.0f, kw_of(rr - 2), ra, rb);
-                const float a0 = ra.s0, a1 = ra.s1, a2 = ra.s2, aw = ra.ws, am = ra.mv, amw = ra.mws;
-                const float b0 = rb.s0, b1 = rb.s1, b2 = rb.s2, bw = rb.ws, bm = rb.mv, bmw = rb.mws;
+                    for (int rr = 5; rr < 7; rr++)
+                        reduce_row<false, true>(rR, rG, rB, rM, base0 + rr * RW, 0.0f, kw_of(rr - 2), ra, rb);
+                    a0 = ra.s0; a1 = ra.s1; a2 = ra.s2; aw = ra.ws; am = ra.mv; amw = ra.mws;
+                    b0 = rb.s0; b1 = rb.s1; b2 = rb.s2; bw = rb.ws; bm = rb.mv; bmw = rb.mws;
+                }
                 const int gi = ox1 + oi;
 #pragma unroll
                 for (int h2 = 0; h2 < 2; h2++) {
@@ -680,13 +779,20 @@ __global__ void __launch_bounds__(256) k_pass2(Pass2Args a, LevelArgs g) {
                     float e0 = 0.0f, e1 = 0.0f, e2 = 0.0f;
                     if (gi >= 0 && gj >= 0 && gi < g.nw && gj < g.nh) {
                         float4 outv = make_float4(-1.0f, 0.0f, 0.0f, 0.0f);
-                        if (ws > 0.0f) {
-                            e0 = s0 / ws;
-                            e1 = s1 / ws;
-                            e2 = s2 / ws;
-                            outv = make_float4(e0, e1, e2, 0.0f);
+                        if (full) {  // division by 256 is exact
+                            e0 = s0 * (1.0f / 256.0f);
+                            e1 = s1 * (1.0f / 256.0f);
+                            e2 = s2 * (1.0f / 256.0f);
+                            outv = make_float4(e0, e1, e2, mv * (1.0f / 256.0f));
+                        } else {
+                            if (ws > 0.0f) {
+                                e0 = s0 / ws;
+                                e1 = s1 / ws;
+                                e2 = s2 / ws;
+                                outv = make_float4(e0, e1, e2, 0.0f);
+                            }
+                            outv.w = mws > 0.0f ? mv / mws : -1.0f;
                         }
-                        outv.w = mws > 0.0f ? mv / mws : -1.0f;
                         if (oi >= 1 && oi <= TW / 2 && oj + h2 >= 1 && oj + h2 <= TH / 2)
                             lvn[gj * g.nw + gi] = outv;
                     }
