@@ -329,6 +329,8 @@ int describe_frame(fg_stack* s, const fg_image_desc* d, const void* px, bool dev
         f->extra = (int32_t)d->extra_channels;
     }
     if (device_ptr) {
+        if ((uintptr_t)px % (uintptr_t)sb != 0)
+            return fail(FG_ERR_INVALID_ARG, "device frame pointer is not aligned to its sample size");
         f->src = px;
     } else {
         size_t nb = desc_bytes(d);

@@ -154,24 +154,10 @@ struct FastTaps {
     bool ok;  // interior of the source: the taps are valid
 };
 
-// Source coordinate of region pixel (gx, gy) by the homography of map_pixel with one reciprocal
-// in place of its two divisions, and the gather of the interior case of samplePixelBilinear
-// (src/remap.zig:788-839, 886-937), 0 <= x < w-1 and 0 <= y < h-1.  Pixels outside that case
-// leave ok = false for the exact path.
+// Gather of the interior case of samplePixelBilinear (src/remap.zig:788-839, 886-937) at
+// (x, y), 0 <= x < w-1 and 0 <= y < h-1, straight from the source in global memory.
 template <typename T>
-__device__ __forceinline__ void gather_fast(const RemapParams& p, const T* __restrict__ src, uint32_t sw, double wlim,
-                                            double hlim, double cx, double cy, double cz, int gy, FastTaps& g) {
-    g.ok = false;
-    const double pyd = (double)(p.roi_top + gy) - p.dcy;
-    const double lx = (cx + p.B[1] * pyd) + p.B[2] * p.neg_d;
-    const double ly = (cy + p.B[4] * pyd) + p.B[5] * p.neg_d;
-    const double lz = (cz + p.B[7] * pyd) + p.B[8] * p.neg_d;
-    const double denom = -lz;
-    if (!(p.legacy ? denom > 1e-6 : fabs(denom) > 1e-6)) return;
-    const double inv = rcp_f64(denom);
-    const double x = p.scx + (p.focal * (lx * inv) + p.csx);
-    const double y = p.scy + (p.focal * (ly * inv) + p.csy);
-    if (!(x >= 0.0 && y >= 0.0 && x < wlim && y < hlim)) return;
+__device__ __forceinline__ void gather_at(const T* __restrict__ src, uint32_t sw, double x, double y, FastTaps& g) {
     const __attribute__((address_space(1))) T* px = as_global(src);
     const double fx0 = floor(x), fy0 = floor(y);
     const uint32_t x0 = (uint32_t)(int)fx0, y0 = (uint32_t)(int)fy0;
@@ -211,10 +197,169 @@ __device__ __forceinline__ bool resolve_fast(const FastTaps& g, double band, uin
     return ok;
 }
 
+// ---- source staging: the footprint of a block's region in one frame's source ----
+// Homography frames have a convex footprint (the image of the region's corners), so its
+// bounding box, read row by row with coalesced dword loads, holds every interior tap.  The box
+// is prefetched into registers one frame ahead and written to LDS over the window-column sums.
+template <typename T>
+struct StageGeom;
+template <>
+struct StageGeom<uint8_t> {
+    static constexpr int NR = 12, NC = 1;  // row slots per wave, dword slots per lane
+};
+template <>
+struct StageGeom<uint16_t> {
+    static constexpr int NR = 10, NC = 2;
+};
+
+struct StageBox {
+    int ok;          // frame staged
+    int x_lo, y_lo;  // first staged source column and row
+    int rows, dw;    // staged rows, dwords per staged row
+};
+
+// Fast-path source coordinate of region pixel (gx, gy); false where the exact path decides.
+__device__ __forceinline__ bool fast_coord(const RemapParams& p, int gx, int gy, double* x, double* y) {
+    const double pxd = (double)(p.roi_left + gx) - p.dcx;
+    const double pyd = (double)(p.roi_top + gy) - p.dcy;
+    const double lx = (p.B[0] * pxd + p.B[1] * pyd) + p.B[2] * p.neg_d;
+    const double ly = (p.B[3] * pxd + p.B[4] * pyd) + p.B[5] * p.neg_d;
+    const double lz = (p.B[6] * pxd + p.B[7] * pyd) + p.B[8] * p.neg_d;
+    const double denom = -lz;
+    if (!(p.legacy ? denom > 1e-6 : fabs(denom) > 1e-6)) return false;
+    const double inv = rcp_f64(denom);
+    *x = p.scx + (p.focal * (lx * inv) + p.csx);
+    *y = p.scy + (p.focal * (ly * inv) + p.csy);
+    return true;
+}
+
+// Bounding box of the four corner coordinates in `cb` (NaN marks an invalid corner).
+template <typename T>
+__device__ __forceinline__ StageBox stage_box(const double* cb, int sw, int sh) {
+    using G = StageGeom<T>;
+    StageBox s{0, 0, 0, 0, 0};
+    double mnx = cb[0], mxx = cb[0], mny = cb[1], mxy = cb[1];
+    bool ok = true;
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+        const double x = cb[2 * c], y = cb[2 * c + 1];
+        ok = ok && x == x && y == y;
+        mnx = fmin(mnx, x);
+        mxx = fmax(mxx, x);
+        mny = fmin(mny, y);
+        mxy = fmax(mxy, y);
+    }
+    if (!ok || !(mnx > -1e6 && mxx < 1e6 && mny > -1e6 && mxy < 1e6)) return s;
+    const int x_lo = max(0, (int)floor(mnx) - 1), x_hi = min(sw - 1, (int)floor(mxx) + 2);
+    const int y_lo = max(0, (int)floor(mny) - 1), y_hi = min(sh - 1, (int)floor(mxy) + 2);
+    if (x_lo > x_hi || y_lo > y_hi) return s;
+    const int rows = y_hi - y_lo + 1;
+    const int dw = (3 * (x_hi - x_lo + 1) * (int)sizeof(T) + 6) / 4;
+    if (rows > 4 * G::NR || dw > 64 * G::NC) return s;
+    s.ok = 1;
+    s.x_lo = x_lo;
+    s.y_lo = y_lo;
+    s.rows = rows;
+    s.dw = dw;
+    return s;
+}
+
+// Byte offset, within the dword-aligned view of `src`, of source row y's first staged byte.
+template <typename T>
+__device__ __forceinline__ uint32_t stage_row_byte(const FrameDesc& f, const StageBox& s, int y, uint32_t lead) {
+    return lead + ((uint32_t)y * f.sw + (uint32_t)s.x_lo) * 3u * (uint32_t)sizeof(T);
+}
+
+// Loads this thread's dword slots of the staged box into registers.  Dwords that straddle the
+// ends of the source buffer are assembled from in-bounds bytes.
+template <typename T>
+__device__ __forceinline__ void stage_load(const FrameDesc& f, const StageBox& s, int lane, int wave,
+                                           uint32_t* pf) {
+    using G = StageGeom<T>;
+    const uintptr_t base = (uintptr_t)f.src;
+    const uint32_t lead = (uint32_t)(base & 3u);
+    const __attribute__((address_space(1))) uint32_t* dwp =
+        (const __attribute__((address_space(1))) uint32_t*)(base - lead);
+    const uint32_t total = f.sw * f.sh * 3u * (uint32_t)sizeof(T) + lead;  // end of the image in the view
+#pragma unroll
+    for (int ir = 0; ir < G::NR; ir++) {
+        const int row = wave + 4 * ir;
+        const uint32_t rb = row < s.rows ? (stage_row_byte<T>(f, s, s.y_lo + row, lead) & ~3u) : 0u;
+#pragma unroll
+        for (int ic = 0; ic < G::NC; ic++) {
+            const int col = lane + 64 * ic;
+            uint32_t v = 0;
+            if (row < s.rows && col < s.dw) {
+                const uint32_t byte = rb + 4u * (uint32_t)col;
+                if (byte >= lead && byte + 4u <= total) {
+                    v = dwp[byte >> 2];
+                } else {
+                    const __attribute__((address_space(1))) uint8_t* bp =
+                        (const __attribute__((address_space(1))) uint8_t*)(base - lead);
+                    for (int i = 0; i < 4; i++)
+                        if (byte + i >= lead && byte + i < total) v |= (uint32_t)bp[byte + i] << (8 * i);
+                }
+            }
+            pf[ir * G::NC + ic] = v;
+        }
+    }
+}
+
+template <typename T>
+__device__ __forceinline__ void stage_store(const StageBox& s, int lane, int wave, const uint32_t* pf,
+                                            uint32_t* stage) {
+    using G = StageGeom<T>;
+#pragma unroll
+    for (int ir = 0; ir < G::NR; ir++) {
+        const int row = wave + 4 * ir;
+#pragma unroll
+        for (int ic = 0; ic < G::NC; ic++) {
+            const int col = lane + 64 * ic;
+            if (row < s.rows && col < s.dw) stage[row * s.dw + col] = pf[ir * G::NC + ic];
+        }
+    }
+}
+
+// gather_fast with the taps read from the staged box; ok = false when the pixel is not in the
+// source interior or, through rounding, a tap falls outside the box.
+template <typename T>
+__device__ __forceinline__ void gather_staged(const FrameDesc& f, const StageBox& s, const uint8_t* stage,
+                                              uint32_t lead, double x, double y, double wlim, double hlim,
+                                              FastTaps& g) {
+    g.ok = false;
+    if (!(x >= 0.0 && y >= 0.0 && x < wlim && y < hlim)) return;
+    const double fx0 = floor(x), fy0 = floor(y);
+    const int x0 = (int)fx0, y0 = (int)fy0;
+    if (x0 < s.x_lo || y0 < s.y_lo || y0 + 1 >= s.y_lo + s.rows) return;
+    const uint32_t xb = (uint32_t)(x0 - s.x_lo) * 3u * (uint32_t)sizeof(T);
+    if (xb + 6u * (uint32_t)sizeof(T) + 3u > 4u * (uint32_t)s.dw) return;
+    const uint32_t r0 = (uint32_t)(y0 - s.y_lo);
+    const uint32_t a0 = r0 * 4u * (uint32_t)s.dw + (stage_row_byte<T>(f, s, y0, lead) & 3u) + xb;
+    const uint32_t a1 = (r0 + 1u) * 4u * (uint32_t)s.dw + (stage_row_byte<T>(f, s, y0 + 1, lead) & 3u) + xb;
+    const double fx = x - fx0, fy = y - fy0;
+    const double omx = 1.0 - fx, omy = 1.0 - fy;
+    g.w00 = omx * omy;
+    g.w10 = fx * omy;
+    g.w01 = omx * fy;
+    g.w11 = fx * fy;
+#pragma unroll
+    for (int c = 0; c < 6; c++) {
+        if (sizeof(T) == 1) {
+            g.t[c] = stage[a0 + c];
+            g.t[6 + c] = stage[a1 + c];
+        } else {
+            g.t[c] = *(const uint16_t*)&stage[a0 + 2 * c];
+            g.t[6 + c] = *(const uint16_t*)&stage[a1 + 2 * c];
+        }
+    }
+    g.ok = true;
+}
+
 template <typename T, int RH>
 __global__ void __launch_bounds__(256) k_pass1(Pass1Args a) {
     using P = Pack<T>;
     using PT = typename P::type;
+    using G = StageGeom<T>;
     extern __shared__ __align__(16) unsigned char smem[];
     const int b = (int)a.window / 2;
     const int th = RH - 2 * b, tw = P1C - 2 * b;
@@ -227,11 +372,13 @@ __global__ void __launch_bounds__(256) k_pass1(Pass1Args a) {
     PT* colour = (PT*)(smem + L.colour);              // [RH][64] hybrid: packed rgb
     double* csum = (double*)(smem + L.csum);          // [th][64] window-column sums of gray
     double* csq = (double*)(smem + L.csq);            // [th][64] ... of the f32 square
+    uint32_t* stage = (uint32_t*)(smem + L.csum);     // staged source box, over csum/csq
     uint8_t* ccnt = smem + L.ccnt;                    // [th][64] ... of supported samples
     float* tS = (float*)(smem + L.tS);                // [th][64] norm sum of the tile
     uint8_t* tU = smem + L.tU;                        // [th][64] union support
     float* tB = (float*)(smem + L.tB);                // [th][64] hybrid: best weight
     PT* tH = (PT*)(smem + L.tH);                      // [th][64] hybrid: its colour
+    __shared__ double corners[8];                     // next frame's region corners in its source
 
     const int W = (int)a.w, H = (int)a.h;
     const int X0 = blockIdx.x * tw, Y0 = blockIdx.y * th;
@@ -246,6 +393,27 @@ __global__ void __launch_bounds__(256) k_pass1(Pass1Args a) {
     const double nf_full = (double)nfull, nf_full1 = (double)(nfull - 1);
     const double r_full = 1.0 / nf_full, r_full1 = 1.0 / nf_full1;
     const double band = sizeof(T) == 1 ? 1e-8 : 4e-6;  // covers the reciprocal coordinates' error
+    const int gx = X0 - b + lane;                     // region column of this lane
+
+    // corner c (threads 0-3) of the region in frame k's source, NaN where not staged
+    auto corner = [&](int k) {
+        if (threadIdx.x >= 4) return;
+        double x = __builtin_nan(""), y = __builtin_nan("");
+        if (k < a.k1) {
+            const FrameDesc& f = a.frames[k];
+            if (f.kind == 0 && f.p.mode == REMAP_HOMOGRAPHY) {
+                const int cx = X0 - b + ((threadIdx.x & 1) ? P1C - 1 : 0);
+                const int cy = Y0 - b + ((threadIdx.x & 2) ? RH - 1 : 0);
+                double sx, sy;
+                if (fast_coord(f.p, cx, cy, &sx, &sy)) {
+                    x = sx;
+                    y = sy;
+                }
+            }
+        }
+        corners[2 * threadIdx.x] = x;
+        corners[2 * threadIdx.x + 1] = y;
+    };
 
 #pragma unroll 1
     for (int r = r0; r < r1; r++) {
@@ -273,70 +441,78 @@ __global__ void __launch_bounds__(256) k_pass1(Pass1Args a) {
         }
     }
 
+    // stage the first frame
+    uint32_t pf[G::NR * G::NC];
+    corner(a.k0);
+    lds_barrier();
+    StageBox box = stage_box<T>(corners, (int)a.frames[a.k0].sw, (int)a.frames[a.k0].sh);
+    if (box.ok) {
+        stage_load<T>(a.frames[a.k0], box, lane, wave, pf);
+        stage_store<T>(box, lane, wave, pf, stage);
+    }
+    lds_barrier();
+
     for (int k = a.k0; k < a.k1; k++) {
         const FrameDesc& f = a.frames[k];
         const size_t slot = (size_t)(k - a.slot_base);
         T* rgba = (T*)a.rgba + slot * fpx * 4;
         float* wout = a.weights + slot * fpx;
-        // remap the region (remapRigidImage); gray (fillAverageFromLoaded) where alpha > 0
-        const int gx = X0 - b + lane;
         const bool fast = f.kind == 0 && f.p.mode == REMAP_HOMOGRAPHY;
-        const RemapParams& p = f.p;
-        double cx = 0.0, cy = 0.0, cz = 0.0;
-        if (fast) {
-            const double pxd = (double)(p.roi_left + gx) - p.dcx;
-            cx = p.B[0] * pxd;
-            cy = p.B[3] * pxd;
-            cz = p.B[6] * pxd;
-        }
+        const uint32_t lead = (uint32_t)((uintptr_t)f.src & 3u);
         const double wlim = (double)f.sw - 1.0, hlim = (double)f.sh - 1.0;
-        // two region rows per step (r, r + 4): both gathers are in flight before either's arithmetic
+        // remap the region (remapRigidImage); gray (fillAverageFromLoaded) where alpha > 0
 #pragma unroll 1
-        for (int r2 = wave; r2 < RH; r2 += 8) {
-            FastTaps gt[2];
-            bool in[2];
-#pragma unroll
-            for (int h2 = 0; h2 < 2; h2++) {
-                const int gy = Y0 - b + r2 + 4 * h2;
-                in[h2] = gx >= 0 && gy >= 0 && gx < W && gy < H;
-                gt[h2].ok = false;
-                if (fast && in[h2]) gather_fast<T>(p, (const T*)f.src, f.sw, wlim, hlim, cx, cy, cz, gy, gt[h2]);
-            }
-#pragma unroll
-            for (int h2 = 0; h2 < 2; h2++) {
-                const int r = r2 + 4 * h2;
-                const int gy = Y0 - b + r;
-                uint32_t o[4] = {0, 0, 0, 0};
-                if (in[h2]) {
-                    if (gt[h2].ok && resolve_fast(gt[h2], band, o)) {
+        for (int r = wave; r < RH; r += 4) {
+            const int gy = Y0 - b + r;
+            uint32_t o[4] = {0, 0, 0, 0};
+            if (gx >= 0 && gy >= 0 && gx < W && gy < H) {
+                bool done = false;
+                if (fast) {
+                    FastTaps t;
+                    t.ok = false;
+                    double sx, sy;
+                    if (fast_coord(f.p, gx, gy, &sx, &sy)) {
+                        if (box.ok)
+                            gather_staged<T>(f, box, (const uint8_t*)stage, lead, sx, sy, wlim, hlim, t);
+                        else if (sx >= 0.0 && sy >= 0.0 && sx < wlim && sy < hlim)
+                            gather_at<T>((const T*)f.src, f.sw, sx, sy, t);
+                    }
+                    if (t.ok && resolve_fast(t, band, o)) {
                         o[3] = SampleMax<T>::i;
-                    } else {
-                        const uint64_t e = frame_rgba_exact<T>(f, gx, gy);
-                        o[0] = (uint32_t)e & 0xffffu;
-                        o[1] = (uint32_t)(e >> 16) & 0xffffu;
-                        o[2] = (uint32_t)(e >> 32) & 0xffffu;
-                        o[3] = (uint32_t)(e >> 48);
+                        done = true;
                     }
                 }
-                float g = -1.0f;
-                if (o[3] > 0) {
-                    const float sc = (float)((o[0] + o[1]) + o[2]);
-                    g = div_markstein(sc, SampleMax<T>::gray3, 1.0f / SampleMax<T>::gray3);
+                if (!done) {
+                    const uint64_t e = frame_rgba_exact<T>(f, gx, gy);
+                    o[0] = (uint32_t)e & 0xffffu;
+                    o[1] = (uint32_t)(e >> 16) & 0xffffu;
+                    o[2] = (uint32_t)(e >> 32) & 0xffffu;
+                    o[3] = (uint32_t)(e >> 48);
                 }
-                const PT c = P::make(o[0], o[1], o[2], o[3]);
-                if (r >= b && r < RH - b && lane >= b && lane < P1C - b && gx < W && gy < H) {
-                    const size_t i = (size_t)gy * W + gx;  // an output pixel of the tile
-                    if (sizeof(T) == 1)
-                        ((uint32_t*)rgba)[i] = *(const uint32_t*)&c;
-                    else
-                        ((uint2*)rgba)[i] = *(const uint2*)&c;
-                }
-                gray[r * P1C + lane] = g;
-                alph[r * P1C + lane] = (uint16_t)o[3];
-                if (hyb) colour[r * P1C + lane] = c;
             }
+            float g = -1.0f;
+            if (o[3] > 0) {
+                const float sc = (float)((o[0] + o[1]) + o[2]);
+                g = div_markstein(sc, SampleMax<T>::gray3, 1.0f / SampleMax<T>::gray3);
+            }
+            const PT c = P::make(o[0], o[1], o[2], o[3]);
+            if (r >= b && r < RH - b && lane >= b && lane < P1C - b && gx < W && gy < H) {
+                const size_t i = (size_t)gy * W + gx;  // an output pixel of the tile
+                if (sizeof(T) == 1)
+                    ((uint32_t*)rgba)[i] = *(const uint32_t*)&c;
+                else
+                    ((uint2*)rgba)[i] = *(const uint2*)&c;
+            }
+            gray[r * P1C + lane] = g;
+            alph[r * P1C + lane] = (uint16_t)o[3];
+            if (hyb) colour[r * P1C + lane] = c;
         }
+        corner(k + 1);
         lds_barrier();
+        // the next frame's source box, in flight until the end of this frame
+        StageBox nbox = k + 1 < a.k1 ? stage_box<T>(corners, (int)a.frames[k + 1].sw, (int)a.frames[k + 1].sh)
+                                     : StageBox{0, 0, 0, 0, 0};
+        if (nbox.ok) stage_load<T>(a.frames[k + 1], nbox, lane, wave, pf);
         // window-column sums of this wave's output rows, sliding down the column
         // (computeRange src/fuse/contrast.zig:201-217: f64 sums of x and of the f32 product x*x)
         if (contrast_ok && r0 < r1) {
@@ -420,6 +596,11 @@ __global__ void __launch_bounds__(256) k_pass1(Pass1Args a) {
             }
         }
         lds_barrier();
+        box = nbox;
+        if (box.ok) {
+            stage_store<T>(box, lane, wave, pf, stage);
+            lds_barrier();
+        }
     }
 
 #pragma unroll 1
