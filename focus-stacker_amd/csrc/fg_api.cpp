@@ -174,6 +174,7 @@ struct fg_stack {
     int sb;
     fg::LevelTable lt;
     bool cached;
+    bool single;          // hardmask / softmask: one pass, no pyramids
     uint32_t added, p1_done, pass2_done;
     bool finished;
     uint32_t src_w, src_h, src_cc, src_extra;  // geometry of the first frame (later frames must match)
@@ -380,9 +381,35 @@ int flush_pass1(fg_stack* s) {
     a.union_sup = s->U;
     a.best = s->best;
     a.hard = s->hard;
+    const bool softmask = s->opts.method == FG_METHOD_SOFTMASK_CONTRAST;
+    if (s->single) {
+        a.slot_base = (int)k0;
+        a.norm_sum = nullptr;
+        a.union_sup = nullptr;
+        if (!softmask) {
+            a.rgba = nullptr;
+            a.weights = nullptr;
+        }
+    }
     {
         PhaseMark pm(s, FG_PHASE_PASS1);
         HIP_TRY(fg::launch_pass1(a, s->stream), "pass1 kernel");
+    }
+    if (softmask) {
+        // blurFiveTapInto + accumulateSoft for the group (src/stack/pipeline.zig:304-308)
+        PhaseMark pm(s, FG_PHASE_PASS2);
+        fg::SoftArgs sa{};
+        sa.sample_bytes = s->sb;
+        sa.k0 = (int)k0;
+        sa.k1 = (int)k1;
+        sa.slot_base = (int)k0;
+        sa.w = s->W;
+        sa.h = s->H;
+        sa.rgba = s->frames;
+        sa.weights = s->weights;
+        sa.wsum = s->S;
+        sa.psum = s->acc;
+        HIP_TRY(fg::launch_soft(sa, s->stream), "soft blend kernel");
     }
     s->p1_done = k1;
     s->stage_used = 0;
@@ -521,8 +548,7 @@ int fg_remap(fg_context* ctx, const fg_image_desc* src, const void* src_px, cons
 int fg_stack_create(fg_context* ctx, const fg_stack_opts* o, fg_stack** out) {
     if (!ctx || !o || !out) return fail(FG_ERR_INVALID_ARG, "null argument");
     *out = nullptr;
-    if (o->method != FG_METHOD_PYRAMID_CONTRAST && o->method != FG_METHOD_HYBRID_PYRAMID_CONTRAST)
-        return fail(FG_ERR_UNSUPPORTED, "method must be pyramid-contrast or hybrid-pyramid-contrast");
+    if (o->method > FG_METHOD_HYBRID_PYRAMID_CONTRAST) return fail(FG_ERR_UNSUPPORTED, "unknown fusion method");
     if (o->contrast_window_size < 3 || (o->contrast_window_size & 1) == 0 || o->contrast_window_size > 31)
         return fail(FG_ERR_INVALID_ARG, "contrast_window_size must be odd and in [3, 31]");
     if (!(o->hybrid_sharpness >= 0.0f && o->hybrid_sharpness <= 1.0f))
@@ -553,9 +579,11 @@ int fg_stack_create(fg_context* ctx, const fg_stack_opts* o, fg_stack** out) {
         s->own_stream = true;
     }
     const bool hybrid = o->method == FG_METHOD_HYBRID_PYRAMID_CONTRAST;
+    const bool hardmask = o->method == FG_METHOD_HARDMASK_CONTRAST;
+    s->single = hardmask || o->method == FG_METHOD_SOFTMASK_CONTRAST;
     const size_t px = (size_t)s->W * s->H;
     const size_t tot = s->lt.total;
-    s->lev_stride = tot - px;  // levels >= 1
+    s->lev_stride = s->single ? 0 : tot - px;  // levels >= 1
     s->contrib_floats = fg::pass2_contrib_floats(&s->lt, o->frame_count);
     size_t fixed = px * 4 + px + tot * 12 + s->contrib_floats * 4 + px * 3 * s->sb + o->frame_count * sizeof(fg::FrameDesc);
     if (hybrid) fixed += px * 4 + 2 * px * 3 * s->sb + 2 * tot * 12;
@@ -565,8 +593,11 @@ int fg_stack_create(fg_context* ctx, const fg_stack_opts* o, fg_stack** out) {
     (void)hipMemGetInfo(&free_b, &total_b);
     float frac = o->memory_fraction >= 0.0f ? o->memory_fraction : ctx->memory_fraction;
     double budget = frac > 0.0f ? (double)frac * (double)free_b : (double)free_b - 1.0e9;
-    s->cached = (double)fixed + (double)o->frame_count * (double)per_frame <= budget;
+    s->cached = !s->single && (double)fixed + (double)o->frame_count * (double)per_frame <= budget;
     s->slots = s->cached ? o->frame_count : 1;
+    // single-pass methods keep no frame past its group: softmask holds a group of frames
+    // between pass 1 and the blend, hardmask only the winners
+    if (s->single) s->slots = hardmask ? 0 : (o->frame_count < kStageSlots ? o->frame_count : kStageSlots);
     void* p;
 #define ALLOC(field, bytes)                      \
     do {                                         \
@@ -586,9 +617,11 @@ int fg_stack_create(fg_context* ctx, const fg_stack_opts* o, fg_stack** out) {
     ALLOC(acc, tot * 3 * sizeof(float));
     ALLOC(contrib, s->contrib_floats * sizeof(float));
     ALLOC(out_dev, px * 3 * s->sb);
-    if (hybrid) {
+    if (hybrid || hardmask) {
         ALLOC(best, px * sizeof(float));
         ALLOC(hard, px * 3 * s->sb);
+    }
+    if (hybrid) {
         ALLOC(soft, px * 3 * s->sb);
         ALLOC(merge_gs, tot * 3 * sizeof(float));
         ALLOC(merge_gh, tot * 3 * sizeof(float));
@@ -625,7 +658,13 @@ static int add_frame(fg_stack* s, const fg_image_desc* d, const void* px, bool d
     if (rc) return rc;
     if (!dev) s->stage_used++;
     s->added++;
-    // uncached stacks keep one frame slot: run pass 1 now; otherwise when the staging is full
+    // uncached stacks keep one frame slot: run pass 1 now; single-pass stacks when their group
+    // of slots is full; otherwise when the host staging is full
+    if (s->single) {
+        if (s->added - s->p1_done >= (s->slots ? s->slots : kStageSlots) || s->stage_used == kStageSlots)
+            return flush_pass1(s);
+        return FG_OK;
+    }
     if (!s->cached || s->stage_used == kStageSlots) return flush_pass1(s);
     return FG_OK;
 }
@@ -638,11 +677,12 @@ int fg_stack_add_frame_device(fg_stack* s, const fg_image_desc* d, const void* p
     return add_frame(s, d, px, true, pose, roi);
 }
 
-int fg_stack_needs_pass2_frames(const fg_stack* s) { return s && !s->cached ? 1 : 0; }
+int fg_stack_needs_pass2_frames(const fg_stack* s) { return s && !s->cached && !s->single ? 1 : 0; }
 
 static int pass2_frame(fg_stack* s, const fg_image_desc* d, const void* px, bool dev, const fg_pose* pose,
                        const fg_rect* roi) {
     if (!s) return fail(FG_ERR_INVALID_ARG, "null stack");
+    if (s->single) return fail(FG_ERR_STATE, "single-pass methods take every frame once; call fg_stack_finish");
     if (s->cached) return fail(FG_ERR_STATE, "frames are cached on the device; call fg_stack_finish");
     if (s->finished) return fail(FG_ERR_STATE, "stack already finished");
     if (s->added != s->opts.frame_count) return fail(FG_ERR_STATE, "pass 1 incomplete");
@@ -697,6 +737,18 @@ static int finish(fg_stack* s) {
     if (rc) return rc;
     rc = flush_pass1(s);
     if (rc) return rc;
+    if (s->single) {
+        // src/stack/pipeline.zig:154-155: hardmask's output is the winners; softmask divides
+        PhaseMark pm(s, FG_PHASE_FINISH);
+        if (s->opts.method == FG_METHOD_HARDMASK_CONTRAST)
+            HIP_TRY(hipMemcpyAsync(s->out_dev, s->hard, (size_t)s->W * s->H * 3 * s->sb, hipMemcpyDeviceToDevice,
+                                   s->stream),
+                    "winners");
+        else
+            HIP_TRY(fg::launch_soft_finalize(s->S, s->acc, s->W, s->H, s->sb, s->out_dev, s->stream), "soft finalize");
+        s->finished = true;
+        return FG_OK;
+    }
     if (s->cached) {
         rc = run_pass2(s, 0, (int)s->opts.frame_count, 0);
         if (rc) return rc;

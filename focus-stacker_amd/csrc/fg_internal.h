@@ -72,8 +72,8 @@ struct Pass1Args {
     int slot_base;            // frame index held in cache slot 0
     int sample_bytes;
     uint32_t w, h, window;
-    void* rgba;               // cache slot j at rgba + j * w*h*4 samples
-    float* weights;           // cache slot j at weights + j * w*h
+    void* rgba;               // cache slot j at rgba + j * w*h*4 samples (nullptr: not kept)
+    float* weights;           // cache slot j at weights + j * w*h (nullptr: not kept)
     float* norm_sum;          // nullptr: weights only (recompute in an uncached second pass)
     uint8_t* union_sup;
     float* best;              // hybrid only, else nullptr
@@ -109,6 +109,21 @@ hipError_t launch_collapse(float* acc, const LevelTable* lt, hipStream_t s);
 hipError_t launch_quantize(const float* acc0, uint32_t w, uint32_t h, int sample_bytes, void* out, hipStream_t s);
 hipError_t launch_zero(void* p, size_t bytes, hipStream_t s);
 hipError_t launch_fill_f32(float* p, size_t n, float v, hipStream_t s);
+
+// softmask-contrast over frames [k0, k1) of the weight / rgba cache slots (slot 0 holds frame
+// slot_base): blur each weight map, add weight and weight * rgb into wsum / psum (planar rgb)
+struct SoftArgs {
+    int sample_bytes;
+    int k0, k1, slot_base;
+    uint32_t w, h;
+    const void* rgba;
+    const float* weights;
+    float* wsum;
+    float* psum;
+};
+hipError_t launch_soft(const SoftArgs& a, hipStream_t s);
+hipError_t launch_soft_finalize(const float* wsum, const float* psum, uint32_t w, uint32_t h, int sample_bytes,
+                                void* out, hipStream_t s);
 
 // hybrid merge (mergeHybridImages): Laplacian pyramids of soft and hard, mix, collapse, quantize
 hipError_t launch_hybrid_merge(const void* soft, const void* hard, int sample_bytes, const LevelTable* lt,

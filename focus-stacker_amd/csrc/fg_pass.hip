@@ -218,13 +218,22 @@ struct StageBox {
     int rows, dw;    // staged rows, dwords per staged row
 };
 
-// Fast-path source coordinate of region pixel (gx, gy); false where the exact path decides.
-__device__ __forceinline__ bool fast_coord(const RemapParams& p, int gx, int gy, double* x, double* y) {
+// Column constants of the fast-path homography for region column gx: B * (x, 0, neg_d).
+struct ColHomog {
+    double cx, cy, cz;
+};
+__device__ __forceinline__ ColHomog col_homog(const RemapParams& p, int gx) {
     const double pxd = (double)(p.roi_left + gx) - p.dcx;
+    return ColHomog{p.B[0] * pxd + p.B[2] * p.neg_d, p.B[3] * pxd + p.B[5] * p.neg_d, p.B[6] * pxd + p.B[8] * p.neg_d};
+}
+
+// Fast-path source coordinate of region pixel (gx, gy) from its column constants; false where the
+// exact path decides.
+__device__ __forceinline__ bool fast_coord(const RemapParams& p, const ColHomog& ch, int gy, double* x, double* y) {
     const double pyd = (double)(p.roi_top + gy) - p.dcy;
-    const double lx = (p.B[0] * pxd + p.B[1] * pyd) + p.B[2] * p.neg_d;
-    const double ly = (p.B[3] * pxd + p.B[4] * pyd) + p.B[5] * p.neg_d;
-    const double lz = (p.B[6] * pxd + p.B[7] * pyd) + p.B[8] * p.neg_d;
+    const double lx = ch.cx + p.B[1] * pyd;
+    const double ly = ch.cy + p.B[4] * pyd;
+    const double lz = ch.cz + p.B[7] * pyd;
     const double denom = -lz;
     if (!(p.legacy ? denom > 1e-6 : fabs(denom) > 1e-6)) return false;
     const double inv = rcp_f64(denom);
@@ -394,6 +403,7 @@ __global__ void __launch_bounds__(256) k_pass1(Pass1Args a) {
     const double r_full = 1.0 / nf_full, r_full1 = 1.0 / nf_full1;
     const double band = sizeof(T) == 1 ? 1e-8 : 4e-6;  // covers the reciprocal coordinates' error
     const int gx = X0 - b + lane;                     // region column of this lane
+    const bool keep_rgba = a.rgba != nullptr, keep_w = a.weights != nullptr;
 
     // corner c (threads 0-3) of the region in frame k's source, NaN where not staged
     auto corner = [&](int k) {
@@ -405,7 +415,7 @@ __global__ void __launch_bounds__(256) k_pass1(Pass1Args a) {
                 const int cx = X0 - b + ((threadIdx.x & 1) ? P1C - 1 : 0);
                 const int cy = Y0 - b + ((threadIdx.x & 2) ? RH - 1 : 0);
                 double sx, sy;
-                if (fast_coord(f.p, cx, cy, &sx, &sy)) {
+                if (fast_coord(f.p, col_homog(f.p, cx), cy, &sx, &sy)) {
                     x = sx;
                     y = sy;
                 }
@@ -455,11 +465,12 @@ __global__ void __launch_bounds__(256) k_pass1(Pass1Args a) {
     for (int k = a.k0; k < a.k1; k++) {
         const FrameDesc& f = a.frames[k];
         const size_t slot = (size_t)(k - a.slot_base);
-        T* rgba = (T*)a.rgba + slot * fpx * 4;
-        float* wout = a.weights + slot * fpx;
+        T* rgba = keep_rgba ? (T*)a.rgba + slot * fpx * 4 : nullptr;
+        float* wout = keep_w ? a.weights + slot * fpx : nullptr;
         const bool fast = f.kind == 0 && f.p.mode == REMAP_HOMOGRAPHY;
         const uint32_t lead = (uint32_t)((uintptr_t)f.src & 3u);
         const double wlim = (double)f.sw - 1.0, hlim = (double)f.sh - 1.0;
+        const ColHomog chm = fast ? col_homog(f.p, gx) : ColHomog{0.0, 0.0, 0.0};
         // remap the region (remapRigidImage); gray (fillAverageFromLoaded) where alpha > 0
 #pragma unroll 1
         for (int r = wave; r < RH; r += 4) {
@@ -471,7 +482,7 @@ __global__ void __launch_bounds__(256) k_pass1(Pass1Args a) {
                     FastTaps t;
                     t.ok = false;
                     double sx, sy;
-                    if (fast_coord(f.p, gx, gy, &sx, &sy)) {
+                    if (fast_coord(f.p, chm, gy, &sx, &sy)) {
                         if (box.ok)
                             gather_staged<T>(f, box, (const uint8_t*)stage, lead, sx, sy, wlim, hlim, t);
                         else if (sx >= 0.0 && sy >= 0.0 && sx < wlim && sy < hlim)
@@ -496,7 +507,7 @@ __global__ void __launch_bounds__(256) k_pass1(Pass1Args a) {
                 g = div_markstein(sc, SampleMax<T>::gray3, 1.0f / SampleMax<T>::gray3);
             }
             const PT c = P::make(o[0], o[1], o[2], o[3]);
-            if (r >= b && r < RH - b && lane >= b && lane < P1C - b && gx < W && gy < H) {
+            if (keep_rgba && r >= b && r < RH - b && lane >= b && lane < P1C - b && gx < W && gy < H) {
                 const size_t i = (size_t)gy * W + gx;  // an output pixel of the tile
                 if (sizeof(T) == 1)
                     ((uint32_t*)rgba)[i] = *(const uint32_t*)&c;
@@ -584,7 +595,7 @@ __global__ void __launch_bounds__(256) k_pass1(Pass1Args a) {
                                  ? 1.0f
                                  : div_markstein((float)alpha, SampleMax<T>::f, 1.0f / SampleMax<T>::f);
             const float wv = stdw * af;  // applySupportInto
-            wout[i] = wv;
+            if (keep_w) wout[i] = wv;
             const int t = r * P1C + lane;
             if (acc) {
                 tS[t] = tS[t] + wv;
@@ -704,21 +715,23 @@ __device__ __forceinline__ void reduce_row(const float* rR, const float* rG, con
                                            float wya, float wyb, RedAcc& a, RedAcc& b) {
     const float KW[5] = {1.0f, 4.0f, 6.0f, 4.0f, 1.0f};
     float tr[5], tg[5], tb[5], tm[5];
+    // 8-byte reads throughout (the fifth tap's pair ends inside the row: base + 5 < RW), which
+    // keeps the stride-2 access pattern free of bank conflicts
     {
-        const float2 u0 = *(const float2*)&rR[base], u1 = *(const float2*)&rR[base + 2];
-        tr[0] = u0.x; tr[1] = u0.y; tr[2] = u1.x; tr[3] = u1.y; tr[4] = rR[base + 4];
+        const float2 u0 = *(const float2*)&rR[base], u1 = *(const float2*)&rR[base + 2], u2 = *(const float2*)&rR[base + 4];
+        tr[0] = u0.x; tr[1] = u0.y; tr[2] = u1.x; tr[3] = u1.y; tr[4] = u2.x;
     }
     {
-        const float2 u0 = *(const float2*)&rG[base], u1 = *(const float2*)&rG[base + 2];
-        tg[0] = u0.x; tg[1] = u0.y; tg[2] = u1.x; tg[3] = u1.y; tg[4] = rG[base + 4];
+        const float2 u0 = *(const float2*)&rG[base], u1 = *(const float2*)&rG[base + 2], u2 = *(const float2*)&rG[base + 4];
+        tg[0] = u0.x; tg[1] = u0.y; tg[2] = u1.x; tg[3] = u1.y; tg[4] = u2.x;
     }
     {
-        const float2 u0 = *(const float2*)&rB[base], u1 = *(const float2*)&rB[base + 2];
-        tb[0] = u0.x; tb[1] = u0.y; tb[2] = u1.x; tb[3] = u1.y; tb[4] = rB[base + 4];
+        const float2 u0 = *(const float2*)&rB[base], u1 = *(const float2*)&rB[base + 2], u2 = *(const float2*)&rB[base + 4];
+        tb[0] = u0.x; tb[1] = u0.y; tb[2] = u1.x; tb[3] = u1.y; tb[4] = u2.x;
     }
     {
-        const float2 u0 = *(const float2*)&rM[base], u1 = *(const float2*)&rM[base + 2];
-        tm[0] = u0.x; tm[1] = u0.y; tm[2] = u1.x; tm[3] = u1.y; tm[4] = rM[base + 4];
+        const float2 u0 = *(const float2*)&rM[base], u1 = *(const float2*)&rM[base + 2], u2 = *(const float2*)&rM[base + 4];
+        tm[0] = u0.x; tm[1] = u0.y; tm[2] = u1.x; tm[3] = u1.y; tm[4] = u2.x;
     }
 #pragma unroll
     for (int kx = 0; kx < 5; kx++) {
@@ -766,7 +779,7 @@ __device__ __forceinline__ void reduce_row_full(const float* rR, const float* rG
 #pragma unroll
     for (int c = 0; c < 4; c++) {
         const float2 u0 = *(const float2*)&ch[c][base], u1 = *(const float2*)&ch[c][base + 2];
-        const float u4 = ch[c][base + 4];
+        const float u4 = (*(const float2*)&ch[c][base + 4]).x;
         float a = s[c];
         a = red_tap<KY * 1>(a, u0.x);
         a = red_tap<KY * 4>(a, u0.y);

@@ -19,7 +19,7 @@ ERRORS = {
     -5: "FG_ERR_FORMAT_MISMATCH", -6: "FG_ERR_UNSUPPORTED", -7: "FG_ERR_DEVICE", -8: "FG_ERR_STATE",
 }
 SAMPLE_U8, SAMPLE_U16 = 0, 1
-METHOD_PYRAMID, METHOD_HYBRID = 2, 3
+METHOD_HARDMASK, METHOD_SOFTMASK, METHOD_PYRAMID, METHOD_HYBRID = 0, 1, 2, 3
 DEBUG = dict(norm_sum=0, union_support=1, weights=2, acc_level=3, mask_level=4, gauss_level=5, soft=6, hard=7)
 # enum fg_phase order; pass2_level0 is the level-0 launch inside pass2
 PHASES = ("ingest", "pass1", "pass2", "finish", "pass2_level0")

@@ -40,7 +40,8 @@ enum fg_status {
 
 enum fg_sample_type { FG_SAMPLE_U8 = 0, FG_SAMPLE_U16 = 1 };   /* image_io.SampleType src/image_io.zig:25-28 */
 
-/* fuse.config.Method (src/fuse/config.zig:10-15); the pyramid methods are served here. */
+/* fuse.config.Method (src/fuse/config.zig:10-15).  hardmask and softmask are the single-pass
+ * methods (src/stack/pipeline.zig:245-312): every frame goes through fg_stack_add_frame once. */
 enum fg_method {
     FG_METHOD_HARDMASK_CONTRAST = 0,
     FG_METHOD_SOFTMASK_CONTRAST = 1,
@@ -79,7 +80,7 @@ typedef struct fg_context_opts {
 } fg_context_opts;
 
 typedef struct fg_stack_opts {
-    uint32_t method;           /* enum fg_method: PYRAMID_CONTRAST or HYBRID_PYRAMID_CONTRAST */
+    uint32_t method;           /* enum fg_method */
     uint32_t contrast_window_size; /* --contrast-window-size, odd, 3..31 (src/fuse/config.zig:295-299) */
     float hybrid_sharpness;    /* --hybrid-sharpness in [0,1] (default 0.35, src/fuse/pyramid.zig:11) */
     uint32_t frame_count;      /* N active frames (the 1/N fallback of normalizeWeightsInto) */

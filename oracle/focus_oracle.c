@@ -981,6 +981,60 @@ void or_update_winners(const void* img, int sb, uint32_t w, uint32_t h, int colo
     }
 }
 
+/* blurFiveTapInto src/fuse/masks.zig:86-105 with horizontalPassRange :151-177 and
+ * verticalPassRange :179-207: clamped (1,4,6,4,1)/16, summed left to right, rows then columns */
+void or_blur_five_tap(uint32_t w, uint32_t h, const R* in, R* scratch, R* out) {
+    for (size_t y = 0; y < h; y++) {
+        const R* r = in + y * w;
+        for (size_t x = 0; x < w; x++) {
+            size_t xm2 = x >= 2 ? x - 2 : 0, xm1 = x >= 1 ? x - 1 : 0;
+            size_t xp1 = x + 1 < w ? x + 1 : w - 1, xp2 = x + 2 < w ? x + 2 : w - 1;
+            scratch[y * w + x] = ((((r[xm2] + (R)4.0 * r[xm1]) + (R)6.0 * r[x]) + (R)4.0 * r[xp1]) + r[xp2]) / (R)16.0;
+        }
+    }
+    for (size_t y = 0; y < h; y++) {
+        size_t ym2 = y >= 2 ? y - 2 : 0, ym1 = y >= 1 ? y - 1 : 0;
+        size_t yp1 = y + 1 < h ? y + 1 : h - 1, yp2 = y + 2 < h ? y + 2 : h - 1;
+        for (size_t x = 0; x < w; x++)
+            out[y * w + x] = ((((scratch[ym2 * w + x] + (R)4.0 * scratch[ym1 * w + x]) + (R)6.0 * scratch[y * w + x]) +
+                               (R)4.0 * scratch[yp1 * w + x]) +
+                              scratch[yp2 * w + x]) /
+                             (R)16.0;
+    }
+}
+
+/* accumulateRange src/fuse/blend.zig:97-148: pixels with weight > 0 add weight and weight*rgb */
+void or_soft_accumulate(const void* img, int sb, uint32_t w, uint32_t h, int color, int extra, const R* wts,
+                        R* ws, R* ps) {
+    size_t n = (size_t)w * h, st = (size_t)(color + extra);
+    for (size_t i = 0; i < n; i++) {
+        R wt = wts[i];
+        if (wt <= 0) continue;
+        ws[i] += wt;
+        for (int c = 0; c < 3; c++) ps[i * 3 + c] += wt * (R)sample_at(img, sb, i * st + c);
+    }
+}
+
+/* finalizeSoft src/fuse/blend.zig:150-186 */
+void or_soft_finalize(const R* ws, const R* ps, uint32_t w, uint32_t h, int sb, void* out) {
+    size_t n = (size_t)w * h;
+    R hi = sb == 1 ? (R)255.0 : (R)65535.0;
+    for (size_t i = 0; i < n; i++) {
+        R wt = ws[i];
+        for (int c = 0; c < 3; c++) {
+            uint32_t q = 0;
+            if (wt > (R)1e-12) {
+                R inv = (R)1.0 / wt;
+                R v = ps[i * 3 + c] * inv + (R)0.5;
+                v = v < (R)0 ? (R)0 : (v > hi ? hi : v);
+                q = (uint32_t)v;
+            }
+            if (sb == 1) ((uint8_t*)out)[i * 3 + c] = (uint8_t)q;
+            else ((uint16_t*)out)[i * 3 + c] = (uint16_t)q;
+        }
+    }
+}
+
 /* ------------------------------------------------------------------------- */
 /* stack drivers — src/stack/pipeline.zig:361-605 and src/fuse/pipeline.zig:254-445   */
 /* ------------------------------------------------------------------------- */
@@ -1032,9 +1086,47 @@ static void accumulate_frame(const void* img, int sb, uint32_t w, uint32_t h, in
     }
 }
 
+/* runRemappedSinglePass src/stack/pipeline.zig:245-312 / runSinglePass src/fuse/pipeline.zig:189-252:
+ * hardmask = winners of the contrast weight times alpha; softmask = support-scaled weights,
+ * blurred, blended, finalised (:135-158) */
+static int single_pass(int n, const void* const* frames, uint32_t w, uint32_t h, int sb, int extra,
+                       const or_fuse_opts* o, void* out, or_fuse_debug* dbg) {
+    size_t count = (size_t)w * h;
+    R* gray = malloc(sizeof(R) * count);
+    R* sup = malloc(sizeof(R) * count);
+    R* wk = malloc(sizeof(R) * count);
+    R* scratch = malloc(sizeof(R) * count);
+    if (o->method == OR_METHOD_HARDMASK) {
+        R* best = malloc(sizeof(R) * count);
+        for (size_t i = 0; i < count; i++) best[i] = -INFINITY;
+        for (int k = 0; k < n; k++) {
+            weight_map(frames[k], sb, w, h, extra, o->window, gray, sup, wk);
+            or_update_winners(frames[k], sb, w, h, 3, extra, wk, best, out);
+        }
+        free(best);
+    } else {
+        R* ws = calloc(count, sizeof(R));
+        R* ps = calloc(count * 3, sizeof(R));
+        for (int k = 0; k < n; k++) {
+            weight_map(frames[k], sb, w, h, extra, o->window, gray, sup, wk);
+            or_apply_support(frames[k], sb, w, h, 3, extra, wk);
+            or_blur_five_tap(w, h, wk, scratch, wk);
+            if (dbg && dbg->weights) memcpy(dbg->weights + count * k, wk, sizeof(R) * count);
+            or_soft_accumulate(frames[k], sb, w, h, 3, extra, wk, ws, ps);
+        }
+        if (dbg && dbg->norm_sum) memcpy(dbg->norm_sum, ws, sizeof(R) * count);
+        or_soft_finalize(ws, ps, w, h, sb, out);
+        free(ws); free(ps);
+    }
+    free(gray); free(sup); free(wk); free(scratch);
+    return 0;
+}
+
 int or_fuse_frames(int n, const void* const* frames, uint32_t w, uint32_t h, int sb, int extra, const or_fuse_opts* o,
                    void* out, or_fuse_debug* dbg) {
     if (n < 1 || w == 0 || h == 0) return -1;
+    if (o->method == OR_METHOD_HARDMASK || o->method == OR_METHOD_SOFTMASK)
+        return single_pass(n, frames, w, h, sb, extra, o, out, dbg);
     size_t count = (size_t)w * h;
     uint32_t levels = or_level_count(w, h);
     if (o->max_levels && o->max_levels < levels) levels = o->max_levels;

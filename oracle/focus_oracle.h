@@ -92,7 +92,7 @@ void or_expand_rgb(uint32_t dw, uint32_t dh, uint32_t sw, uint32_t sh, const or_
 void or_fill_rgb_base(const void* img, int sample_bytes, uint32_t w, uint32_t h, int color, int extra, or_real* dst);
 
 /* Whole-stack fusion of already-remapped (or pre-aligned) frames. */
-enum { OR_METHOD_PYRAMID = 0, OR_METHOD_HYBRID = 1 };
+enum { OR_METHOD_PYRAMID = 0, OR_METHOD_HYBRID = 1, OR_METHOD_HARDMASK = 2, OR_METHOD_SOFTMASK = 3 };
 typedef struct {
     int method;               /* OR_METHOD_* */
     uint32_t window;          /* contrast window, odd >= 3 (reference default 5) */
@@ -124,6 +124,14 @@ int or_fuse_frames(int n, const void* const* frames, uint32_t w, uint32_t h, int
 /* Remap every source under its pose into `roi`, then fuse (the focus_stack_zig path). */
 int or_stack(int n, const void* const* sources, uint32_t src_w, uint32_t src_h, int sample_bytes,
              const or_pose* poses, const int32_t* roi, const or_fuse_opts* opts, void* out, or_fuse_debug* dbg);
+
+/* Single-pass pieces: blurFiveTapInto (src/fuse/masks.zig:86-207), in == out allowed, and the
+ * soft blend accumulate / finalize (src/fuse/blend.zig:51-186). */
+void or_blur_five_tap(uint32_t w, uint32_t h, const or_real* in, or_real* scratch, or_real* out);
+void or_soft_accumulate(const void* img, int sample_bytes, uint32_t w, uint32_t h, int color, int extra,
+                        const or_real* weights, or_real* weight_sums, or_real* pixel_sums);
+void or_soft_finalize(const or_real* weight_sums, const or_real* pixel_sums, uint32_t w, uint32_t h, int sample_bytes,
+                      void* out);
 
 /* Hybrid pieces (a18). */
 void or_update_winners(const void* img, int sample_bytes, uint32_t w, uint32_t h, int color, int extra,

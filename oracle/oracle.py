@@ -23,6 +23,10 @@ POSE_FIELDS = (
 )
 
 
+
+# OR_METHOD_* of focus_oracle.h
+METHOD_PYRAMID, METHOD_HYBRID, METHOD_HARDMASK, METHOD_SOFTMASK = 0, 1, 2, 3
+
 class Pose(C.Structure):
     _fields_ = [(f, C.c_double) for f in POSE_FIELDS]
 
@@ -121,6 +125,9 @@ class Oracle:
         L.or_stack.argtypes = [i32, vp, u32, u32, i32, vp, vp, C.POINTER(FuseOpts), vp, C.POINTER(FuseDebug)]
         L.or_stack.restype = i32
         L.or_update_winners.argtypes = [vp, i32, u32, u32, i32, i32, vp, vp, vp]
+        L.or_blur_five_tap.argtypes = [u32, u32, vp, vp, vp]
+        L.or_soft_accumulate.argtypes = [vp, i32, u32, u32, i32, i32, vp, vp, vp]
+        L.or_soft_finalize.argtypes = [vp, vp, u32, u32, i32, vp]
         L.or_merge_hybrid.argtypes = [vp, vp, i32, u32, u32, u32, u32, C.c_float, u32, vp]
         L.or_collapse.argtypes = [u32, u32, u32, vp, i32, vp]
 
@@ -244,6 +251,29 @@ class Oracle:
         raw = np.ascontiguousarray(raw_weights, self.real)
         self.lib.or_update_winners(_ptr(img), img.dtype.itemsize, w, h, cc - extra, extra, _ptr(raw), _ptr(best),
                                    _ptr(out))
+
+    def blur_five_tap(self, img):
+        src = np.ascontiguousarray(img, self.real)
+        h, w = src.shape
+        scratch = np.zeros_like(src)
+        out = np.zeros_like(src)
+        self.lib.or_blur_five_tap(w, h, _ptr(src), _ptr(scratch), _ptr(out))
+        return out
+
+    def soft_accumulate(self, img, weights, weight_sums, pixel_sums, extra=1):
+        img = np.ascontiguousarray(img)
+        h, w, cc = img.shape
+        wt = np.ascontiguousarray(weights, self.real)
+        self.lib.or_soft_accumulate(_ptr(img), img.dtype.itemsize, w, h, cc - extra, extra, _ptr(wt),
+                                    _ptr(weight_sums), _ptr(pixel_sums))
+
+    def soft_finalize(self, weight_sums, pixel_sums, dtype=np.uint8):
+        h, w = weight_sums.shape
+        out = np.zeros((h, w, 3), dtype)
+        self.lib.or_soft_finalize(_ptr(np.ascontiguousarray(weight_sums, self.real)),
+                                  _ptr(np.ascontiguousarray(pixel_sums, self.real)), w, h, np.dtype(dtype).itemsize,
+                                  _ptr(out))
+        return out
 
     def merge_hybrid(self, soft, hard, mix, start=2, count=4, max_levels=0):
         soft = np.ascontiguousarray(soft)

@@ -278,3 +278,47 @@ def test_errors_surface_as_status_codes(ctx, fg):
     s.close()
     with pytest.raises(fg.FgError):
         fg.Stack(ctx, 2, 32, 24, window=4)
+
+
+SINGLE_CASES = {
+    "hardmask_u8": ("hardmask", dict(frames=5, width=150, height=110)),
+    "softmask_u8": ("softmask", dict(frames=5, width=150, height=110)),
+    "softmask_groups_u8": ("softmask", dict(frames=11, width=97, height=70)),   # more frames than one group
+    "softmask_u16": ("softmask", dict(frames=3, width=120, height=90, dtype="u16")),
+    "hardmask_u16": ("hardmask", dict(frames=3, width=120, height=90, dtype="u16")),
+}
+
+
+@pytest.mark.parametrize("case", list(SINGLE_CASES))
+def test_single_pass_methods_match_oracle(ctx, fg, oracle, case):
+    """hardmask-contrast and softmask-contrast (src/stack/pipeline.zig:245-312)."""
+    from oracle import make_pose, METHOD_HARDMASK, METHOD_SOFTMASK
+    method, kw = SINGLE_CASES[case]
+    spec, frames, poses = _synthetic(kw)
+    opo = [make_pose(**p) for p in poses]
+    roi = oracle.common_overlap_roi([(spec.width, spec.height)] * spec.frames, opo)
+    fm, om = (fg.METHOD_HARDMASK, METHOD_HARDMASK) if method == "hardmask" else (fg.METHOD_SOFTMASK, METHOD_SOFTMASK)
+    s, out = _run_stack(fg, ctx, frames, poses, roi, method=fm)
+    assert not s.needs_pass2_frames()
+    ref = oracle.stack(frames, opo, roi, method=om)
+    if method == "hardmask":
+        # winners flip only where two frames' weights tie to the last ulp
+        assert (out != ref).any(axis=2).mean() < 1e-3
+    else:
+        _img_close(out, ref)
+    s.close()
+
+
+def test_softmask_aligned_frames(ctx, fg, oracle):
+    from oracle import METHOD_SOFTMASK
+    g = np.random.default_rng(5)
+    frames = [g.integers(0, 256, (60, 80, 4)).astype(np.uint8) for _ in range(4)]
+    for f in frames:
+        f[..., 3] = 255
+        f[:3, :, 3] = 0
+    s = fg.Stack(ctx, 4, 80, 60, fg.SAMPLE_U8, method=fg.METHOD_SOFTMASK)
+    for f in frames:
+        s.add_frame(f, extra=1)
+    out = s.finish()
+    _img_close(out, oracle.fuse_frames(frames, method=METHOD_SOFTMASK))
+    s.close()

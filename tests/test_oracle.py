@@ -461,3 +461,75 @@ def test_f32_oracle_within_tolerance_of_f64(oracle, oracle64):
     b = oracle64.stack(src, poses, roi).astype(int)
     d = np.abs(a - b)
     assert d.max() <= 1 and d.mean() < 0.05
+
+
+# ---- single-pass modes (hardmask-contrast, softmask-contrast) ----
+
+def test_blur_keeps_uniform_weights(oracle):
+    # src/fuse/masks.zig:209-226
+    out = oracle.blur_five_tap(np.ones((3, 3), np.float32))
+    assert np.allclose(out, 1.0, atol=1e-6)
+
+
+@pytest.mark.parametrize("h,w", [(1, 1), (2, 7), (9, 5), (17, 23)])
+def test_blur_brute_force(oracle, h, w):
+    """Edge padding and whole-array slices instead of per-index clamping; same f32 sum order."""
+    g = np.random.default_rng(h * 100 + w)
+    x = g.random((h, w), dtype=np.float32) * np.float32(3)
+    p = np.pad(x, ((0, 0), (2, 2)), mode="edge")
+    hb = ((((p[:, 0:w] + np.float32(4) * p[:, 1:w + 1]) + np.float32(6) * p[:, 2:w + 2]) +
+           np.float32(4) * p[:, 3:w + 3]) + p[:, 4:w + 4]) / np.float32(16)
+    q = np.pad(hb, ((2, 2), (0, 0)), mode="edge")
+    vb = ((((q[0:h] + np.float32(4) * q[1:h + 1]) + np.float32(6) * q[2:h + 2]) +
+           np.float32(4) * q[3:h + 3]) + q[4:h + 4]) / np.float32(16)
+    assert np.array_equal(oracle.blur_five_tap(x), vb)
+
+
+def test_soft_blend_known_answer(oracle):
+    # src/fuse/blend.zig:403-436: weights 1 and 3 over (10,20,30) and (110,120,130)
+    ws = np.zeros((1, 1), np.float32)
+    ps = np.zeros((1, 1, 3), np.float32)
+    oracle.soft_accumulate(np.array([[[10, 20, 30]]], np.uint8), np.array([[1.0]], np.float32), ws, ps, extra=0)
+    oracle.soft_accumulate(np.array([[[110, 120, 130]]], np.uint8), np.array([[3.0]], np.float32), ws, ps, extra=0)
+    assert oracle.soft_finalize(ws, ps).ravel().tolist() == [85, 95, 105]
+
+
+def test_soft_blend_skips_nonpositive_and_empty(oracle):
+    ws = np.zeros((1, 2), np.float32)
+    ps = np.zeros((1, 2, 3), np.float32)
+    img = np.array([[[200, 100, 50, 255], [7, 8, 9, 255]]], np.uint8)
+    oracle.soft_accumulate(img, np.array([[-1.0, 0.0]], np.float32), ws, ps)
+    assert ws.tolist() == [[0.0, 0.0]]
+    assert oracle.soft_finalize(ws, ps).ravel().tolist() == [0] * 6
+
+
+def test_single_pass_modes_on_identical_frames(oracle):
+    """N identical frames: every method returns the frame (hardmask: frame 0 wins every tie;
+    softmask: the weighted mean of equal colours, wherever some weight is positive)."""
+    f = _frames(1, 30, 41, 5)[0]
+    frames = [f, f.copy(), f.copy()]
+    from oracle import METHOD_HARDMASK, METHOD_SOFTMASK
+    hard = oracle.fuse_frames(frames, method=METHOD_HARDMASK)
+    assert np.array_equal(hard, f[..., :3])
+    soft = oracle.fuse_frames(frames, method=METHOD_SOFTMASK)
+    keep = soft.sum(axis=2) > 0
+    assert keep.mean() > 0.5
+    assert np.abs(soft.astype(int) - f[..., :3].astype(int))[keep].max() <= 1
+
+
+def test_softmask_matches_numpy_restatement(oracle64):
+    """Softmask end to end against a numpy composition of the pinned pieces (f64 build)."""
+    from oracle import METHOD_SOFTMASK
+    frames = _frames(3, 24, 19, 11)
+    out = oracle64.fuse_frames(frames, method=METHOD_SOFTMASK)
+    h, w = frames[0].shape[:2]
+    ws = np.zeros((h, w)); ps = np.zeros((h, w, 3))
+    for f in frames:
+        wk = oracle64.contrast(oracle64.gray(f), oracle64.binary_support(f), 5)
+        wk = wk * (f[..., 3] / 255.0)
+        wk = oracle64.blur_five_tap(wk)
+        pos = wk > 0
+        ws[pos] += wk[pos]
+        ps[pos] += wk[pos][:, None] * f[..., :3][pos]
+    ref = np.where(ws[..., None] > 1e-12, np.clip(np.floor(ps / np.where(ws > 0, ws, 1)[..., None] + 0.5), 0, 255), 0)
+    assert np.abs(out.astype(int) - ref.astype(int)).max() <= 1
