@@ -80,6 +80,7 @@ def main():
     ap.add_argument("--frames", type=int, default=None, help="override the frame count (quick runs)")
     ap.add_argument("--levels", type=int, default=0, help="0 = full depth (reference)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-host", action="store_true", help="skip the host-fed (PCIe-inclusive) step")
     ap.add_argument("--profile-only", action="store_true", help="one profiled step (for rocprofv3 runs)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -206,6 +207,31 @@ def main():
                      "achieved_GBs": round(pipe_gbs, 1), "frac": round(pipe_gbs / PEAK_HBM_GBS, 4)},
         "phases_ms_per_step": {k: round(v[0] / args.steps, 3) for k, v in ph.items()},
     }
+    if not args.no_host:
+        # the same stack fed from pageable host memory through fg_stack_add_frame: PCIe-inclusive
+        # (uploads overlap pass 1 of earlier frames on the stack's upload stream)
+        frames_host = [f.cpu().numpy() for f in frames]
+        hsteps = 2
+        stack.reset()
+        for k in range(n):
+            stack.add_frame(frames_host[k], fposes[k], roi)
+        stack.finish_device(out.data_ptr())
+        stack.sync()
+        t0 = time.perf_counter()
+        for _ in range(hsteps):
+            stack.reset()
+            for k in range(n):
+                stack.add_frame(frames_host[k], fposes[k], roi)
+            stack.finish_device(out.data_ptr())
+        stack.sync()
+        hms = (time.perf_counter() - t0) * 1000.0 / hsteps
+        src_gb = n * frames_host[0].nbytes / 1e9
+        line["pcie_inclusive"] = {"value": round(frame_px / (hms / 1000.0) / 1e6, 2), "unit": "MPix/s",
+                                  "ms_per_step": round(hms, 3), "source_GB_per_step": round(src_gb, 3),
+                                  "h2d_GBs": round(src_gb / (hms / 1000.0), 2),
+                                  "note": "frames in pageable host memory, fg_stack_add_frame; not the value"}
+        del frames_host
+        print(f"bench: host-fed {hms:.1f} ms/step", file=sys.stderr, flush=True)
     if not args.no_cpu_baseline:
         del frames[2:]
         torch.cuda.synchronize()

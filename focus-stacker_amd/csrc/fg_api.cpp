@@ -199,6 +199,12 @@ struct fg_stack {
     std::vector<uint8_t*> stage;  // device copies of host-fed source frames awaiting pass 1
     size_t stage_bytes;
     uint32_t stage_used;
+    // uploads of host-fed frames run on their own stream: the compute stream waits for a slot's
+    // upload, the next upload into a slot waits until pass 1 has read it
+    hipStream_t cstream;
+    hipEvent_t up_ev[8];
+    hipEvent_t free_ev[8];
+    bool free_rec[8];
     float* accDbg;
     uint8_t* out_dev;
     float* merge_gs;
@@ -228,18 +234,29 @@ hipEvent_t take_event(fg_stack* s) {
 struct PhaseMark {
     fg_stack* s;
     int phase;
+    hipStream_t st;
     hipEvent_t a;
-    PhaseMark(fg_stack* st, int ph) : s(st), phase(ph), a(nullptr) {
-        if (ph >= 0 && s->profiling && (a = take_event(s))) (void)hipEventRecord(a, s->stream);
+    PhaseMark(fg_stack* sk, int ph, hipStream_t on = nullptr) : s(sk), phase(ph), st(on ? on : sk->stream), a(nullptr) {
+        if (ph >= 0 && s->profiling && (a = take_event(s))) (void)hipEventRecord(a, st);
     }
     ~PhaseMark() {
         if (!a) return;
         hipEvent_t b = take_event(s);
         if (!b) return;
-        (void)hipEventRecord(b, s->stream);
+        (void)hipEventRecord(b, st);
         s->ev[phase].emplace_back(a, b);
     }
 };
+
+// Marks the staging slots [0, n) as read once the work queued so far on the compute stream is done.
+int release_slots(fg_stack* s, uint32_t n) {
+    if (!s->cstream) return FG_OK;
+    for (uint32_t j = 0; j < n && j < kStageSlots; j++) {
+        HIP_TRY(hipEventRecord(s->free_ev[j], s->stream), "slot event");
+        s->free_rec[j] = true;
+    }
+    return FG_OK;
+}
 
 void recycle_events(fg_stack* s) {
     for (int k = 0; k < FG_PHASE_COUNT; k++) {
@@ -268,7 +285,15 @@ int dev_alloc(fg_stack* s, size_t bytes, void** out) {
 
 void free_stack(fg_stack* s) {
     if (!s) return;
+    if (s->cstream) (void)hipStreamSynchronize(s->cstream);
     if (s->stream) (void)hipStreamSynchronize(s->stream);
+    if (s->cstream) {
+        for (int j = 0; j < 8; j++) {
+            (void)hipEventDestroy(s->up_ev[j]);
+            (void)hipEventDestroy(s->free_ev[j]);
+        }
+        (void)hipStreamDestroy(s->cstream);
+    }
     recycle_events(s);
     for (hipEvent_t e : s->ev_pool) (void)hipEventDestroy(e);
     for (void* p : s->allocs) (void)hipFree(p);
@@ -337,6 +362,14 @@ int describe_frame(fg_stack* s, const fg_image_desc* d, const void* px, bool dev
         size_t nb = desc_bytes(d);
         if (s->stage.empty() || nb > s->stage_bytes) {
             if (!s->stage.empty()) return fail(FG_ERR_SIZE_MISMATCH, "frame larger than the first frame");
+            if (!s->cstream) {
+                HIP_TRY(hipStreamCreateWithFlags(&s->cstream, hipStreamNonBlocking), "hipStreamCreate");
+                for (uint32_t j = 0; j < kStageSlots; j++) {
+                    HIP_TRY(hipEventCreateWithFlags(&s->up_ev[j], hipEventDisableTiming), "hipEventCreate");
+                    HIP_TRY(hipEventCreateWithFlags(&s->free_ev[j], hipEventDisableTiming), "hipEventCreate");
+                    s->free_rec[j] = false;
+                }
+            }
             for (uint32_t j = 0; j < kStageSlots; j++) {
                 void* p = nullptr;
                 rc = dev_alloc(s, nb, &p);
@@ -345,8 +378,18 @@ int describe_frame(fg_stack* s, const fg_image_desc* d, const void* px, bool dev
             }
             s->stage_bytes = nb;
         }
-        uint8_t* slot = s->stage[s->stage_used];
-        HIP_TRY(hipMemcpyAsync(slot, px, nb, hipMemcpyHostToDevice, s->stream), "upload frame");
+        const uint32_t j = s->stage_used;
+        uint8_t* slot = s->stage[j];
+        if (s->free_rec[j]) HIP_TRY(hipStreamWaitEvent(s->cstream, s->free_ev[j], 0), "slot wait");
+        {
+            PhaseMark pm(s, FG_PHASE_INGEST, s->cstream);
+            HIP_TRY(hipMemcpyAsync(slot, px, nb, hipMemcpyHostToDevice, s->cstream), "upload frame");
+        }
+        HIP_TRY(hipEventRecord(s->up_ev[j], s->cstream), "upload event");
+        HIP_TRY(hipStreamWaitEvent(s->stream, s->up_ev[j], 0), "upload wait");
+        // the caller may release px on return: wait for this upload (pass 1 of earlier frames
+        // keeps running on the compute stream meanwhile)
+        HIP_TRY(hipEventSynchronize(s->up_ev[j]), "upload");
         f->src = slot;
     }
     if (!s->have_src) {
@@ -412,8 +455,9 @@ int flush_pass1(fg_stack* s) {
         HIP_TRY(fg::launch_soft(sa, s->stream), "soft blend kernel");
     }
     s->p1_done = k1;
+    int rc = release_slots(s, s->stage_used);
     s->stage_used = 0;
-    return FG_OK;
+    return rc;
 }
 
 fg::Pass2Args pass2_args(fg_stack* s, int k0, int k1, int slot_base) {
@@ -651,10 +695,7 @@ static int add_frame(fg_stack* s, const fg_image_desc* d, const void* px, bool d
     int rc = set_device(s->ctx->device);
     if (rc) return rc;
     const uint32_t k = s->added;
-    {
-        PhaseMark pm(s, dev ? -1 : FG_PHASE_INGEST);   // device frames enqueue no copy
-        rc = describe_frame(s, d, px, dev, pose, roi, &s->fdesc[k]);
-    }
+    rc = describe_frame(s, d, px, dev, pose, roi, &s->fdesc[k]);
     if (rc) return rc;
     if (!dev) s->stage_used++;
     s->added++;
@@ -690,10 +731,7 @@ static int pass2_frame(fg_stack* s, const fg_image_desc* d, const void* px, bool
     int rc = set_device(s->ctx->device);
     if (rc) return rc;
     const uint32_t k = s->pass2_done;
-    {
-        PhaseMark pm(s, FG_PHASE_INGEST);
-        rc = describe_frame(s, d, px, dev, pose, roi, &s->fdesc[k]);
-    }
+    rc = describe_frame(s, d, px, dev, pose, roi, &s->fdesc[k]);
     if (rc) return rc;
     // the frame's RGBA and weight map are rebuilt as in pass 1 (src/stack/pipeline.zig:566-572),
     // leaving the final norm sum and union support untouched
@@ -714,6 +752,8 @@ static int pass2_frame(fg_stack* s, const fg_image_desc* d, const void* px, bool
         PhaseMark pm(s, FG_PHASE_PASS1);
         HIP_TRY(fg::launch_pass1(a, s->stream), "pass1 kernel");
     }
+    rc = release_slots(s, dev ? 0 : 1);
+    if (rc) return rc;
     s->stage_used = 0;
     rc = run_pass2(s, (int)k, (int)k + 1, (int)k);
     if (rc) return rc;
@@ -811,6 +851,7 @@ void* fg_stack_stream(const fg_stack* s) { return s ? (void*)s->stream : nullptr
 
 int fg_stack_sync(fg_stack* s) {
     if (!s) return fail(FG_ERR_INVALID_ARG, "null stack");
+    if (s->cstream) HIP_TRY(hipStreamSynchronize(s->cstream), "sync");
     HIP_TRY(hipStreamSynchronize(s->stream), "sync");
     return FG_OK;
 }

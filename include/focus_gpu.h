@@ -170,7 +170,7 @@ int fg_stack_reset(fg_stack* stack);
 /* Device time per phase, from HIP events recorded on the stack's stream while profiling is
  * on, accumulated over every stack run since fg_stack_set_profiling(stack, 1).  ms[k] is the
  * total and launches[k] the number of bracketed regions, for k < FG_PHASE_COUNT:
- *   INGEST        host-frame uploads;
+ *   INGEST        host-frame uploads (on the stack's upload stream, overlapping the others);
  *   PASS1         the pass-1 launch (remap + weights + norm sum), one per flush;
  *   PASS2         every pass-2 launch of a stack (all levels);
  *   FINISH        collapse, quantize and the hybrid merge;
