@@ -231,7 +231,25 @@ def main():
                                   "h2d_GBs": round(src_gb / (hms / 1000.0), 2),
                                   "note": "frames in pageable host memory, fg_stack_add_frame; not the value"}
         del frames_host
-        print(f"bench: host-fed {hms:.1f} ms/step", file=sys.stderr, flush=True)
+        # the same from page-locked buffers (fg_host_alloc), as a decoder writing into them would
+        pinned = []
+        for f in frames:
+            pa = fg.PinnedArray(tuple(f.shape), np.uint8)
+            pa.array[...] = f.cpu().numpy()
+            pinned.append(pa)
+        t0 = time.perf_counter()
+        for _ in range(hsteps):
+            stack.reset()
+            for k in range(n):
+                stack.add_frame(pinned[k].array, fposes[k], roi)
+            stack.finish_device(out.data_ptr())
+        stack.sync()
+        pms = (time.perf_counter() - t0) * 1000.0 / hsteps
+        line["pcie_inclusive"]["pinned"] = {"value": round(frame_px / (pms / 1000.0) / 1e6, 2), "ms_per_step": round(pms, 3),
+                                            "h2d_GBs": round(src_gb / (pms / 1000.0), 2)}
+        for pa in pinned:
+            pa.close()
+        print(f"bench: host-fed {hms:.1f} ms/step, pinned {pms:.1f} ms/step", file=sys.stderr, flush=True)
     if not args.no_cpu_baseline:
         del frames[2:]
         torch.cuda.synchronize()

@@ -501,6 +501,21 @@ int run_pass2(fg_stack* s, int k0, int k1, int slot_base) {
 extern "C" {
 
 const char* fg_last_error(void) { return g_last_error.c_str(); }
+
+void* fg_host_alloc(size_t bytes) {
+    void* p = nullptr;
+    hipError_t e = hipHostMalloc(&p, bytes ? bytes : 1, hipHostMallocDefault);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        fail_hip(e, "hipHostMalloc");
+        return nullptr;
+    }
+    return p;
+}
+
+void fg_host_free(void* p) {
+    if (p) (void)hipHostFree(p);
+}
 const char* fg_version(void) { return "focus_gpu 0.1 (gfx950)"; }
 
 int fg_context_create(const fg_context_opts* opts, fg_context** out) {

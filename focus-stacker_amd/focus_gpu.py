@@ -120,8 +120,30 @@ def lib() -> C.CDLL:
     L.fg_stack_set_profiling.argtypes = [vp, i32]
     L.fg_stack_phase_times.argtypes = [vp, vp, vp]
     L.fg_fuse_stack.argtypes = [vp, u32, vp, vp, vp, C.POINTER(Rect), C.POINTER(StackOpts), vp]
+    L.fg_host_alloc.argtypes = [C.c_size_t]
+    L.fg_host_alloc.restype = vp
+    L.fg_host_free.argtypes = [vp]
+    L.fg_host_free.restype = None
     _lib = L
     return L
+
+
+class PinnedArray:
+    """A numpy array over page-locked host memory from fg_host_alloc (freed with close())."""
+
+    def __init__(self, shape, dtype=np.uint8):
+        n = int(np.prod(shape)) * np.dtype(dtype).itemsize
+        self.ptr = lib().fg_host_alloc(n)
+        if not self.ptr:
+            raise FgError(-2, lib().fg_last_error().decode())
+        buf = (C.c_uint8 * n).from_address(self.ptr)
+        self.array = np.frombuffer(buf, dtype=dtype, count=int(np.prod(shape))).reshape(shape)
+
+    def close(self):
+        if self.ptr:
+            self.array = None
+            lib().fg_host_free(self.ptr)
+            self.ptr = None
 
 
 def check(rc: int) -> int:

@@ -103,6 +103,12 @@ enum fg_debug_buffer {
     FG_DEBUG_HARD = 7          /* samples W*H*3: hybrid hard-mask image */
 };
 
+/* Page-locked host memory for frame pixels (image_io.loadImage's pixel buffers,
+ * src/image_io.zig:160-205, can be allocated here): fg_stack_add_frame* then uploads at full PCIe
+ * rate.  fg_host_free releases it.  Returns NULL on failure. */
+void* fg_host_alloc(size_t bytes);
+void fg_host_free(void* p);
+
 /* Message of the last failed call on this thread ("" if none). */
 const char* fg_last_error(void);
 /* Library version string. */

@@ -322,3 +322,21 @@ def test_softmask_aligned_frames(ctx, fg, oracle):
     out = s.finish()
     _img_close(out, oracle.fuse_frames(frames, method=METHOD_SOFTMASK))
     s.close()
+
+
+def test_pinned_host_frames_match_pageable(ctx, fg, oracle):
+    """Frames in fg_host_alloc memory go through the same upload path and give the same image."""
+    spec, frames, poses = _synthetic(dict(frames=3, width=140, height=100))
+    from oracle import make_pose
+    opo = [make_pose(**p) for p in poses]
+    roi = oracle.common_overlap_roi([(spec.width, spec.height)] * spec.frames, opo)
+    _, out1 = _run_stack(fg, ctx, frames, poses, roi)
+    pinned = []
+    for f in frames:
+        pa = fg.PinnedArray(f.shape, f.dtype)
+        pa.array[...] = f
+        pinned.append(pa)
+    _, out2 = _run_stack(fg, ctx, [p.array for p in pinned], poses, roi)
+    for p in pinned:
+        p.close()
+    assert np.array_equal(out1, out2)
