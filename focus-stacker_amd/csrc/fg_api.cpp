@@ -12,6 +12,7 @@
 #include <vector>
 
 #include "../../include/focus_gpu.h"
+#include "fg_host.h"
 #include "fg_internal.h"
 
 namespace {
@@ -516,6 +517,39 @@ void* fg_host_alloc(size_t bytes) {
 void fg_host_free(void* p) {
     if (p) (void)hipHostFree(p);
 }
+
+void* fg_device_alloc(fg_context* ctx, size_t bytes) {
+    if (!ctx || set_device(ctx->device)) return nullptr;
+    void* p = nullptr;
+    hipError_t e = hipMalloc(&p, bytes ? bytes : 1);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        fail_hip(e, "hipMalloc");
+        return nullptr;
+    }
+    return p;
+}
+
+void fg_device_free(fg_context* ctx, void* p) {
+    if (!ctx || !p || set_device(ctx->device)) return;
+    (void)hipFree(p);
+}
+
+int fg_copy_to_device(fg_context* ctx, void* dst, const void* src, size_t bytes) {
+    if (!ctx || !dst || !src) return fail(FG_ERR_INVALID_ARG, "null argument");
+    int rc = set_device(ctx->device);
+    if (rc) return rc;
+    HIP_TRY(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice), "copy to device");
+    return FG_OK;
+}
+
+int fg_copy_to_host(fg_context* ctx, void* dst, const void* src, size_t bytes) {
+    if (!ctx || !dst || !src) return fail(FG_ERR_INVALID_ARG, "null argument");
+    int rc = set_device(ctx->device);
+    if (rc) return rc;
+    HIP_TRY(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost), "copy to host");
+    return FG_OK;
+}
 const char* fg_version(void) { return "focus_gpu 0.1 (gfx950)"; }
 
 int fg_context_create(const fg_context_opts* opts, fg_context** out) {
@@ -604,9 +638,44 @@ int fg_remap(fg_context* ctx, const fg_image_desc* src, const void* src_px, cons
     return FG_OK;
 }
 
-int fg_stack_create(fg_context* ctx, const fg_stack_opts* o, fg_stack** out) {
-    if (!ctx || !o || !out) return fail(FG_ERR_INVALID_ARG, "null argument");
-    *out = nullptr;
+}  // extern "C"
+
+// Sizes of a stack's device buffers; `budget` decides whether every frame is cached.
+struct StackPlan {
+    fg::LevelTable lt;
+    size_t px, tot, lev_stride, contrib_floats;
+    bool hybrid, hardmask, single, cached;
+    uint32_t slots;
+    size_t bytes;
+};
+
+static StackPlan plan_stack(const fg_stack_opts& o, double budget) {
+    StackPlan pl{};
+    const size_t sb = o.sample_type == FG_SAMPLE_U16 ? 2 : 1;
+    fg::make_levels(o.width, o.height, o.max_levels, &pl.lt);
+    pl.hybrid = o.method == FG_METHOD_HYBRID_PYRAMID_CONTRAST;
+    pl.hardmask = o.method == FG_METHOD_HARDMASK_CONTRAST;
+    pl.single = pl.hardmask || o.method == FG_METHOD_SOFTMASK_CONTRAST;
+    pl.px = (size_t)o.width * o.height;
+    pl.tot = pl.lt.total;
+    pl.lev_stride = pl.single ? 0 : pl.tot - pl.px;  // levels >= 1
+    pl.contrib_floats = fg::pass2_contrib_floats(&pl.lt, o.frame_count);
+    size_t fixed = pl.px * 4 + pl.px + pl.tot * 12 + pl.contrib_floats * 4 + pl.px * 3 * sb +
+                   o.frame_count * sizeof(fg::FrameDesc);
+    if (pl.hybrid || pl.hardmask) fixed += pl.px * 4 + pl.px * 3 * sb;
+    if (pl.hybrid) fixed += pl.px * 3 * sb + 2 * pl.tot * 12;
+    if (o.debug_frame >= 0) fixed += pl.tot * 12;
+    const size_t per_frame = pl.px * 4 * sb + pl.px * 4 + pl.lev_stride * sizeof(float4);
+    pl.cached = !pl.single && (double)fixed + (double)o.frame_count * (double)per_frame <= budget;
+    pl.slots = pl.cached ? o.frame_count : 1;
+    // single-pass methods keep no frame past its group: softmask holds a group of frames
+    // between pass 1 and the blend, hardmask only the winners
+    if (pl.single) pl.slots = pl.hardmask ? 0 : (o.frame_count < kStageSlots ? o.frame_count : kStageSlots);
+    pl.bytes = fixed + pl.slots * per_frame;
+    return pl;
+}
+
+static int check_stack_opts(const fg_stack_opts* o) {
     if (o->method > FG_METHOD_HYBRID_PYRAMID_CONTRAST) return fail(FG_ERR_UNSUPPORTED, "unknown fusion method");
     if (o->contrast_window_size < 3 || (o->contrast_window_size & 1) == 0 || o->contrast_window_size > 31)
         return fail(FG_ERR_INVALID_ARG, "contrast_window_size must be odd and in [3, 31]");
@@ -615,8 +684,33 @@ int fg_stack_create(fg_context* ctx, const fg_stack_opts* o, fg_stack** out) {
     if (o->frame_count == 0 || o->width == 0 || o->height == 0) return fail(FG_ERR_INVALID_ARG, "empty stack");
     if (o->sample_type != FG_SAMPLE_U8 && o->sample_type != FG_SAMPLE_U16)
         return fail(FG_ERR_UNSUPPORTED, "sample_type must be u8 or u16");
-    int rc = set_device(ctx->device);
+    return FG_OK;
+}
+
+int fg_context_device(const fg_context* ctx) { return ctx ? ctx->device : 0; }
+int fg_set_error(int code, const char* msg) { return fail(code, msg ? msg : ""); }
+
+size_t fg_plan_stack_bytes(const fg_stack_opts* o, double budget, int* cached) {
+    if (check_stack_opts(o)) return 0;
+    StackPlan pl = plan_stack(*o, budget);
+    if (cached) *cached = pl.cached ? 1 : 0;
+    return pl.bytes;
+}
+
+int fg_stack_create_budget(fg_context* ctx, const fg_stack_opts* o, double budget, fg_stack** out) {
+    if (!ctx || !o || !out) return fail(FG_ERR_INVALID_ARG, "null argument");
+    *out = nullptr;
+    int rc = check_stack_opts(o);
     if (rc) return rc;
+    rc = set_device(ctx->device);
+    if (rc) return rc;
+    if (budget < 0.0) {
+        size_t free_b = 0, total_b = 0;
+        (void)hipMemGetInfo(&free_b, &total_b);
+        float frac = o->memory_fraction >= 0.0f ? o->memory_fraction : ctx->memory_fraction;
+        budget = frac > 0.0f ? (double)frac * (double)free_b : (double)free_b - 1.0e9;
+    }
+    const StackPlan pl = plan_stack(*o, budget);
     fg_stack* s = new (std::nothrow) fg_stack();
     if (!s) return fail(FG_ERR_OOM_HOST, "stack allocation");
     s->ctx = ctx;
@@ -624,7 +718,7 @@ int fg_stack_create(fg_context* ctx, const fg_stack_opts* o, fg_stack** out) {
     s->W = o->width;
     s->H = o->height;
     s->sb = o->sample_type == FG_SAMPLE_U16 ? 2 : 1;
-    fg::make_levels(s->W, s->H, o->max_levels, &s->lt);
+    s->lt = pl.lt;
     s->fdesc.resize(o->frame_count);
     if (o->stream) {
         s->stream = (hipStream_t)o->stream;
@@ -637,26 +731,12 @@ int fg_stack_create(fg_context* ctx, const fg_stack_opts* o, fg_stack** out) {
         }
         s->own_stream = true;
     }
-    const bool hybrid = o->method == FG_METHOD_HYBRID_PYRAMID_CONTRAST;
-    const bool hardmask = o->method == FG_METHOD_HARDMASK_CONTRAST;
-    s->single = hardmask || o->method == FG_METHOD_SOFTMASK_CONTRAST;
-    const size_t px = (size_t)s->W * s->H;
-    const size_t tot = s->lt.total;
-    s->lev_stride = s->single ? 0 : tot - px;  // levels >= 1
-    s->contrib_floats = fg::pass2_contrib_floats(&s->lt, o->frame_count);
-    size_t fixed = px * 4 + px + tot * 12 + s->contrib_floats * 4 + px * 3 * s->sb + o->frame_count * sizeof(fg::FrameDesc);
-    if (hybrid) fixed += px * 4 + 2 * px * 3 * s->sb + 2 * tot * 12;
-    if (o->debug_frame >= 0) fixed += tot * 12;
-    const size_t per_frame = frame_bytes(s) + weight_bytes(s) + s->lev_stride * sizeof(float4);
-    size_t free_b = 0, total_b = 0;
-    (void)hipMemGetInfo(&free_b, &total_b);
-    float frac = o->memory_fraction >= 0.0f ? o->memory_fraction : ctx->memory_fraction;
-    double budget = frac > 0.0f ? (double)frac * (double)free_b : (double)free_b - 1.0e9;
-    s->cached = !s->single && (double)fixed + (double)o->frame_count * (double)per_frame <= budget;
-    s->slots = s->cached ? o->frame_count : 1;
-    // single-pass methods keep no frame past its group: softmask holds a group of frames
-    // between pass 1 and the blend, hardmask only the winners
-    if (s->single) s->slots = hardmask ? 0 : (o->frame_count < kStageSlots ? o->frame_count : kStageSlots);
+    const size_t px = pl.px, tot = pl.tot;
+    s->single = pl.single;
+    s->lev_stride = pl.lev_stride;
+    s->contrib_floats = pl.contrib_floats;
+    s->cached = pl.cached;
+    s->slots = pl.slots;
     void* p;
 #define ALLOC(field, bytes)                      \
     do {                                         \
@@ -676,11 +756,11 @@ int fg_stack_create(fg_context* ctx, const fg_stack_opts* o, fg_stack** out) {
     ALLOC(acc, tot * 3 * sizeof(float));
     ALLOC(contrib, s->contrib_floats * sizeof(float));
     ALLOC(out_dev, px * 3 * s->sb);
-    if (hybrid || hardmask) {
+    if (pl.hybrid || pl.hardmask) {
         ALLOC(best, px * sizeof(float));
         ALLOC(hard, px * 3 * s->sb);
     }
-    if (hybrid) {
+    if (pl.hybrid) {
         ALLOC(soft, px * 3 * s->sb);
         ALLOC(merge_gs, tot * 3 * sizeof(float));
         ALLOC(merge_gh, tot * 3 * sizeof(float));
@@ -694,6 +774,12 @@ int fg_stack_create(fg_context* ctx, const fg_stack_opts* o, fg_stack** out) {
     }
     *out = s;
     return FG_OK;
+}
+
+extern "C" {
+
+int fg_stack_create(fg_context* ctx, const fg_stack_opts* o, fg_stack** out) {
+    return fg_stack_create_budget(ctx, o, -1.0, out);
 }
 
 void fg_stack_destroy(fg_stack* s) {

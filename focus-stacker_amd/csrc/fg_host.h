@@ -1,0 +1,17 @@
+// Host-side internals shared by the C-ABI layer (fg_api.cpp) and the stack server (fg_server.cpp).
+#pragma once
+
+#include <stddef.h>
+
+#include "../../include/focus_gpu.h"
+
+// Device bytes of a stack with these options when `budget` bytes are available; *cached tells
+// whether every frame then stays on the device.  0 for invalid options.
+size_t fg_plan_stack_bytes(const fg_stack_opts* o, double budget, int* cached);
+// fg_stack_create with the caching decision taken against `budget` bytes (< 0: the options'
+// memory fraction of free HBM).
+int fg_stack_create_budget(fg_context* ctx, const fg_stack_opts* o, double budget, fg_stack** out);
+// HIP device of a context.
+int fg_context_device(const fg_context* ctx);
+// Sets the calling thread's fg_last_error() message and returns `code`.
+int fg_set_error(int code, const char* msg);

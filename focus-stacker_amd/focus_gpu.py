@@ -61,6 +61,23 @@ class StackOpts(C.Structure):
                 ("debug_frame", C.c_int32), ("stream", C.c_void_p)]
 
 
+class ServerOpts(C.Structure):
+    _fields_ = [("slots", C.c_int32), ("memory_fraction", C.c_float), ("pool_per_slot", C.c_int32)]
+
+
+class Request(C.Structure):
+    _fields_ = [("frame_count", C.c_uint32), ("descs", C.c_void_p), ("frames", C.c_void_p), ("poses", C.c_void_p),
+                ("roi", C.c_void_p), ("method", C.c_uint32), ("contrast_window_size", C.c_uint32),
+                ("hybrid_sharpness", C.c_float), ("max_levels", C.c_uint32), ("out", C.c_void_p)]
+
+
+class RequestStatus(C.Structure):
+    _fields_ = [("state", C.c_int32), ("queue_ms", C.c_double), ("run_ms", C.c_double), ("total_ms", C.c_double)]
+
+
+REQ_QUEUED, REQ_RUNNING, REQ_DONE = 0, 1, 2
+
+
 def pose(d: dict | None = None, **kw) -> Pose:
     p = Pose()
     p.base_hfov_degrees = 50.0
@@ -120,6 +137,17 @@ def lib() -> C.CDLL:
     L.fg_stack_set_profiling.argtypes = [vp, i32]
     L.fg_stack_phase_times.argtypes = [vp, vp, vp]
     L.fg_fuse_stack.argtypes = [vp, u32, vp, vp, vp, C.POINTER(Rect), C.POINTER(StackOpts), vp]
+    L.fg_server_create.argtypes = [vp, C.POINTER(ServerOpts), C.POINTER(vp)]
+    L.fg_server_destroy.argtypes = [vp]
+    L.fg_server_destroy.restype = None
+    L.fg_server_submit.argtypes = [vp, C.POINTER(Request), C.POINTER(C.c_uint64)]
+    L.fg_server_wait.argtypes = [vp, C.c_uint64, C.c_double, C.POINTER(RequestStatus)]
+    L.fg_device_alloc.argtypes = [vp, C.c_size_t]
+    L.fg_device_alloc.restype = vp
+    L.fg_device_free.argtypes = [vp, vp]
+    L.fg_device_free.restype = None
+    L.fg_copy_to_device.argtypes = [vp, vp, vp, C.c_size_t]
+    L.fg_copy_to_host.argtypes = [vp, vp, vp, C.c_size_t]
     L.fg_host_alloc.argtypes = [C.c_size_t]
     L.fg_host_alloc.restype = vp
     L.fg_host_free.argtypes = [vp]
@@ -171,6 +199,27 @@ class Context:
 
     def __exit__(self, *a):
         self.close()
+
+    def alloc(self, nbytes: int) -> int:
+        p = lib().fg_device_alloc(self.h, nbytes)
+        if not p:
+            raise FgError(-3, lib().fg_last_error().decode())
+        return p
+
+    def free(self, d_ptr: int):
+        lib().fg_device_free(self.h, d_ptr)
+
+    def upload(self, a: np.ndarray) -> int:
+        """A device copy of `a` (freed with free())."""
+        a = np.ascontiguousarray(a)
+        p = self.alloc(a.nbytes)
+        check(lib().fg_copy_to_device(self.h, p, _p(a), a.nbytes))
+        return p
+
+    def download(self, d_ptr: int, shape, dtype=np.uint8) -> np.ndarray:
+        out = np.empty(shape, dtype)
+        check(lib().fg_copy_to_host(self.h, _p(out), d_ptr, out.nbytes))
+        return out
 
     def remap(self, src: np.ndarray, p: Pose, roi=None) -> np.ndarray:
         src = np.ascontiguousarray(src)
@@ -246,6 +295,12 @@ class Stack:
                                          C.byref(p) if p is not None else None,
                                          C.byref(rr) if rr is not None else None))
 
+    def pass2_frame_device(self, d_px: int, d: ImageDesc, p: Pose | None = None, roi=None):
+        rr = rect(roi)
+        check(lib().fg_stack_pass2_frame_device(self.h_, C.byref(d), C.c_void_p(d_px),
+                                                C.byref(p) if p is not None else None,
+                                                C.byref(rr) if rr is not None else None))
+
     def finish(self) -> np.ndarray:
         out = np.zeros((self.h, self.w, 3), self.dtype)
         check(lib().fg_stack_finish(self.h_, _p(out)))
@@ -294,3 +349,45 @@ class Stack:
 
     def sync(self):
         check(lib().fg_stack_sync(self.h_))
+
+
+class Server:
+    """Continuous batching of whole stacks (fg_server_*): requests carry device-resident frames and
+    a device output buffer; submit() queues, wait() returns the request's status and latencies."""
+
+    def __init__(self, ctx: Context, slots: int = 4, memory_fraction: float = 0.0, pool_per_slot: int = 2):
+        self.h_ = C.c_void_p()
+        check(lib().fg_server_create(ctx.h, C.byref(ServerOpts(slots, memory_fraction, pool_per_slot)),
+                                     C.byref(self.h_)))
+        self._keep = {}
+
+    def submit(self, frames_dev, desc: ImageDesc, out_dev: int, poses=None, roi=None, method: int = METHOD_PYRAMID,
+               window: int = 5, sharpness: float = 0.35, max_levels: int = 0) -> int:
+        n = len(frames_dev)
+        descs = (ImageDesc * n)(*([desc] * n))
+        ptrs = (C.c_void_p * n)(*frames_dev)
+        pz = (Pose * n)(*poses) if poses is not None else None
+        rr = rect(roi)
+        q = Request(n, C.cast(descs, C.c_void_p), C.cast(ptrs, C.c_void_p),
+                    C.cast(pz, C.c_void_p) if pz is not None else None,
+                    C.cast(C.pointer(rr), C.c_void_p) if rr is not None else None,
+                    method, window, sharpness, max_levels, out_dev)
+        t = C.c_uint64()
+        check(lib().fg_server_submit(self.h_, C.byref(q), C.byref(t)))
+        return t.value
+
+    def wait(self, ticket: int, timeout_ms: float = -1.0) -> dict:
+        st = RequestStatus()
+        check(lib().fg_server_wait(self.h_, ticket, timeout_ms, C.byref(st)))
+        return {"state": st.state, "queue_ms": st.queue_ms, "run_ms": st.run_ms, "total_ms": st.total_ms}
+
+    def close(self):
+        if self.h_:
+            lib().fg_server_destroy(self.h_)
+            self.h_ = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()

@@ -109,6 +109,14 @@ enum fg_debug_buffer {
 void* fg_host_alloc(size_t bytes);
 void fg_host_free(void* p);
 
+/* Device memory on the context's GPU, for callers without a HIP binding (the Zig host) that keep
+ * frames device-resident for the *_device entry points and the server.  The copies are
+ * synchronous.  fg_device_alloc returns NULL on failure. */
+void* fg_device_alloc(fg_context* ctx, size_t bytes);
+void fg_device_free(fg_context* ctx, void* d_ptr);
+int fg_copy_to_device(fg_context* ctx, void* d_dst, const void* src, size_t bytes);
+int fg_copy_to_host(fg_context* ctx, void* dst, const void* d_src, size_t bytes);
+
 /* Message of the last failed call on this thread ("" if none). */
 const char* fg_last_error(void);
 /* Library version string. */
@@ -192,6 +200,53 @@ int fg_stack_phase_times(fg_stack* stack, double* ms, uint32_t* launches);
  * aligned frames, roi may be NULL). */
 int fg_fuse_stack(fg_context* ctx, uint32_t n, const fg_image_desc* descs, const void* const* pxs,
                   const fg_pose* poses, const fg_rect* roi, const fg_stack_opts* opts, void* out_px);
+
+/* ---- Serving: continuous batching of whole stacks ----
+ * No counterpart in the reference, which fuses one stack per process.  A server keeps up to
+ * `slots` stacks in flight, each on its own HIP stream, so the kernels of uneven stacks overlap
+ * on the device.  Queued requests are admitted, in submission order, whenever a slot and the
+ * memory budget (memory_fraction x free HBM at creation, the --memory-fraction of
+ * src/memory_budget.zig:3) allow.  Frames stay device-resident from submission to the fused
+ * output.  Idle stacks are pooled per slot for same-geometry requests.  A stack's result does not
+ * depend on what else runs: every request computes exactly what fg_fuse_stack computes. */
+typedef struct fg_server fg_server;
+
+typedef struct fg_server_opts {
+    int32_t slots;             /* stacks in flight at once (0 = 4) */
+    float memory_fraction;     /* share of free HBM the server may hold (0 = the context's; <= 0 there: all but 1 GB) */
+    int32_t pool_per_slot;     /* idle stacks kept per slot for reuse (0 = 2, < 0 = none) */
+} fg_server_opts;
+
+typedef struct fg_request {
+    uint32_t frame_count;
+    const fg_image_desc* descs;    /* per frame */
+    const void* const* frames;     /* per frame: device pointer, valid until the request is done */
+    const fg_pose* poses;          /* per frame, or NULL for aligned frames */
+    const fg_rect* roi;            /* NULL = full frame */
+    uint32_t method;               /* enum fg_method */
+    uint32_t contrast_window_size;
+    float hybrid_sharpness;
+    uint32_t max_levels;
+    void* out;                     /* device buffer: W*H*3 samples of the stack's sample type */
+} fg_request;
+
+enum fg_request_state { FG_REQ_QUEUED = 0, FG_REQ_RUNNING = 1, FG_REQ_DONE = 2 };
+
+typedef struct fg_request_status {
+    int32_t state;                 /* enum fg_request_state, or a negative FG_ERR_* of a failed request */
+    double queue_ms;               /* submission to admission */
+    double run_ms;                 /* admission to completion on the device */
+    double total_ms;               /* submission to completion */
+} fg_request_status;
+
+int fg_server_create(fg_context* ctx, const fg_server_opts* opts, fg_server** out);
+/* Finishes every submitted request, then releases the server. */
+void fg_server_destroy(fg_server* server);
+/* Queues a copy of *req (the descriptor, pose and frame-pointer arrays are copied). */
+int fg_server_submit(fg_server* server, const fg_request* req, uint64_t* ticket);
+/* Waits up to timeout_ms (< 0: no limit) for a request.  A done or failed request fills *st and
+ * releases its ticket; on timeout *st holds QUEUED or RUNNING. */
+int fg_server_wait(fg_server* server, uint64_t ticket, double timeout_ms, fg_request_status* st);
 
 #ifdef __cplusplus
 }
