@@ -18,6 +18,8 @@ import os
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 for p in (os.path.join(ROOT, "focus-stacker_amd"), os.path.join(ROOT, "oracle")):
     if p not in sys.path:
@@ -72,6 +74,150 @@ def cpu_baseline(frames_np, poses, roi, budget_s=30.0):
                       f"{dt:.1f} s"}
 
 
+MIX_SIZES = [(640, 480), (1920, 1080), (4000, 3000), (6000, 4000)]
+
+
+def make_mix(fg, synth, torch, dev, n_stacks, seed=7):
+    """Uneven stacks: frame counts U{4..40}, sizes from MIX_SIZES, poses of the 1024x768 config scaled
+    to the size.  Frames are smooth random images made on the GPU, one pool of 40 per size (each
+    stack takes the first n of its size's pool under its own poses)."""
+    rng = np.random.default_rng(seed)
+    g = torch.Generator(device=dev)
+    g.manual_seed(seed)
+    pools = {}
+    for w, h in MIX_SIZES:
+        fr = []
+        for _ in range(40):
+            small = torch.randint(0, 256, (h // 16 + 2, w // 16 + 2, 3), device=dev, dtype=torch.uint8, generator=g)
+            big = torch.nn.functional.interpolate(small.permute(2, 0, 1)[None].float(), size=(h, w), mode="bicubic",
+                                                  align_corners=False)
+            fr.append(big[0].permute(1, 2, 0).round().clamp(0, 255).to(torch.uint8).contiguous())
+        pools[(w, h)] = fr
+    stacks = []
+    for i in range(n_stacks):
+        n = int(rng.integers(4, 41))
+        w, h = MIX_SIZES[int(rng.integers(0, len(MIX_SIZES)))]
+        spec = synth.config_spec(0, stack_id=i, frames=n, width=w, height=h, shift=3.0 * w / 1024.0)
+        poses = [fg.pose(p) for p in synth.make_poses(spec)]
+        roi = fg.common_overlap_roi([(w, h)] * n, poses)
+        stacks.append(dict(n=n, w=w, h=h, frames=[f.data_ptr() for f in pools[(w, h)][:n]], poses=poses, roi=roi))
+    return stacks, pools
+
+
+def _roi_px(s):
+    return (s["roi"][2] - s["roi"][0]) * (s["roi"][3] - s["roi"][1])
+
+
+def run_batch(args):
+    """Uneven stacks: the plain per-stack calls (create, add, finish, destroy; one after another)
+    against the server (all submitted at once, --slots in flight)."""
+    import torch
+    import focus_gpu as fg
+    import synth
+    dev = torch.device("cuda:0")
+    stacks, pools = make_mix(fg, synth, torch, dev, args.stacks)
+    torch.cuda.synchronize()
+    ctx = fg.Context(0, memory_fraction=0.8)
+    px = sum(s["n"] * _roi_px(s) for s in stacks)
+    outs_a, outs_b = [], []
+    for s in stacks:
+        rw, rh = s["roi"][2] - s["roi"][0], s["roi"][3] - s["roi"][1]
+        outs_a.append(torch.empty((rh, rw, 3), dtype=torch.uint8, device=dev))
+        outs_b.append(torch.zeros((rh, rw, 3), dtype=torch.uint8, device=dev))
+
+    def plain(sub):
+        for s, o in sub:
+            rw, rh = s["roi"][2] - s["roi"][0], s["roi"][3] - s["roi"][1]
+            st = fg.Stack(ctx, s["n"], rw, rh, fg.SAMPLE_U8)
+            desc = fg.ImageDesc(s["w"], s["h"], 3, 0, fg.SAMPLE_U8)
+            for f, p in zip(s["frames"], s["poses"]):
+                st.add_frame_device(f, desc, p, s["roi"])
+            st.finish_device(o.data_ptr())
+            st.sync()
+            st.close()
+
+    print(f"bench: {len(stacks)} stacks, {px / 1e9:.2f} GPix; warmup", file=sys.stderr, flush=True)
+    plain(list(zip(stacks[:4], outs_a[:4])))
+    t0 = time.perf_counter()
+    plain(list(zip(stacks, outs_a)))
+    t_plain = time.perf_counter() - t0
+    print(f"bench: plain {t_plain:.3f} s", file=sys.stderr, flush=True)
+    with fg.Server(ctx, slots=args.slots, memory_fraction=0.8) as srv:
+        def desc_of(s):
+            return fg.ImageDesc(s["w"], s["h"], 3, 0, fg.SAMPLE_U8)
+        for s, o in zip(stacks[:4], outs_b[:4]):
+            srv.wait(srv.submit(s["frames"], desc_of(s), o.data_ptr(), poses=s["poses"], roi=s["roi"]))
+        t0 = time.perf_counter()
+        tickets = [srv.submit(s["frames"], desc_of(s), o.data_ptr(), poses=s["poses"], roi=s["roi"])
+                   for s, o in zip(stacks, outs_b)]
+        status = [srv.wait(t) for t in tickets]
+        t_srv = time.perf_counter() - t0
+    torch.cuda.synchronize()
+    same = all(bool(torch.equal(a, b)) for a, b in zip(outs_a, outs_b))
+    lat = [st["total_ms"] for st in status]
+    line = {"metric": "stacks/s, uneven stacks (4-40 frames of 640x480..6000x4000 u8), server vs plain per-stack",
+            "value": round(len(stacks) / t_srv, 3), "unit": "stacks/s", "n_gpus": 1, "steps": 1, "warmup": 1,
+            "ms_per_step": round(t_srv * 1000.0, 3), "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": "u8", "data": "synthetic",
+            "config": {"workload": f"{len(stacks)} uneven stacks, pyramid-contrast, full depth", "slots": args.slots,
+                       "frame_pixels": px},
+            "plain_stacks_per_s": round(len(stacks) / t_plain, 3), "speedup": round(t_plain / t_srv, 3),
+            "fused_MPix_per_s": {"server": round(px / t_srv / 1e6, 1), "plain": round(px / t_plain / 1e6, 1)},
+            "latency_ms": {"p50": round(pct(lat, 50), 3), "p99": round(pct(lat, 99), 3)},
+            "identical_to_plain": same}
+    print(json.dumps(line), flush=True)
+
+
+def run_serve(args):
+    """Poisson arrivals of stacks drawn from the uneven mix at several rates; admission by the
+    device-memory budget; per-rate throughput and p50/p99 latency."""
+    import torch
+    import focus_gpu as fg
+    import synth
+    dev = torch.device("cuda:0")
+    stacks, pools = make_mix(fg, synth, torch, dev, args.stacks)
+    torch.cuda.synchronize()
+    ctx = fg.Context(0, memory_fraction=0.8)
+    outs = {}
+    for i, s in enumerate(stacks):
+        rw, rh = s["roi"][2] - s["roi"][0], s["roi"][3] - s["roi"][1]
+        outs[i] = torch.empty((rh, rw, 3), dtype=torch.uint8, device=dev)
+    rng = np.random.default_rng(11)
+    rates = []
+    with fg.Server(ctx, slots=args.slots, memory_fraction=0.8) as srv:
+        for rate in args.rates:
+            nreq = args.requests
+            gaps = rng.exponential(1.0 / rate, nreq)
+            pick = rng.integers(0, len(stacks), nreq)
+            t0 = time.perf_counter()
+            due = t0
+            tickets = []
+            for gap, i in zip(gaps, pick):
+                due += gap
+                while time.perf_counter() < due:
+                    time.sleep(min(0.0005, max(0.0, due - time.perf_counter())))
+                s = stacks[int(i)]
+                d = fg.ImageDesc(s["w"], s["h"], 3, 0, fg.SAMPLE_U8)
+                tickets.append(srv.submit(s["frames"], d, outs[int(i)].data_ptr(), poses=s["poses"], roi=s["roi"]))
+            status = [srv.wait(t) for t in tickets]
+            el = time.perf_counter() - t0
+            lat = [st["total_ms"] for st in status]
+            qs = [st["queue_ms"] for st in status]
+            px = sum(stacks[int(i)]["n"] * _roi_px(stacks[int(i)]) for i in pick)
+            r = {"rate": rate, "requests": nreq, "stacks_per_s": round(nreq / el, 3),
+                 "fused_MPix_per_s": round(px / el / 1e6, 1), "p50_ms": round(pct(lat, 50), 3),
+                 "p99_ms": round(pct(lat, 99), 3), "queue_p50_ms": round(pct(qs, 50), 3)}
+            rates.append(r)
+            print(f"bench: serve {r}", file=sys.stderr, flush=True)
+    best = max(rates, key=lambda r: r["stacks_per_s"])
+    line = {"metric": "served stacks/s and p50/p99 latency under Poisson arrivals (uneven stack mix)",
+            "value": best["stacks_per_s"], "unit": "stacks/s", "n_gpus": 1, "steps": len(rates), "warmup": 0,
+            "ms_per_step": None, "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u8",
+            "data": "synthetic", "config": {"workload": f"Poisson arrivals over {len(stacks)} uneven stacks",
+                                            "slots": args.slots, "memory_fraction": 0.8}, "rates": rates}
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -82,12 +228,22 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-host", action="store_true", help="skip the host-fed (PCIe-inclusive) step")
     ap.add_argument("--profile-only", action="store_true", help="one profiled step (for rocprofv3 runs)")
+    ap.add_argument("--workload", choices=("stack", "batch", "serve"), default="stack",
+                    help="stack: the 30 x 7952x5304 stack (default); batch / serve: uneven stack mixes")
+    ap.add_argument("--stacks", type=int, default=64)
+    ap.add_argument("--slots", type=int, default=4)
+    ap.add_argument("--requests", type=int, default=48)
+    ap.add_argument("--rates", type=float, nargs="+", default=[2.0, 8.0, 32.0])
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: runs on one GPU only; a stack would shard across GPUs by frames with an "
               "all-reduce of the norm sum and the accumulator pyramid, which is not built "
               "(--gpus must be 1)", file=sys.stderr)
         sys.exit(2)
+    if args.workload == "batch":
+        return run_batch(args)
+    if args.workload == "serve":
+        return run_serve(args)
 
     import numpy as np
     import torch

@@ -183,6 +183,7 @@ struct fg_stack {
     std::vector<fg::FrameDesc> fdesc;
     // device memory
     std::vector<void*> allocs;
+    fg_dev_allocator ext;   // buffers from an external allocator when ext.alloc is set
     fg::FrameDesc* d_fdesc;
     uint8_t* frames;      // RGBA cache slots
     float* weights;       // weight cache slots
@@ -274,6 +275,13 @@ size_t weight_bytes(const fg_stack* s) { return (size_t)s->W * s->H * sizeof(flo
 
 int dev_alloc(fg_stack* s, size_t bytes, void** out) {
     void* p = nullptr;
+    if (s->ext.alloc) {
+        p = s->ext.alloc(s->ext.user, bytes ? bytes : 256);
+        if (!p) return fail(FG_ERR_OOM_DEVICE, "arena allocation of " + std::to_string(bytes) + " bytes failed");
+        s->allocs.push_back(p);
+        *out = p;
+        return FG_OK;
+    }
     hipError_t e = hipMalloc(&p, bytes ? bytes : 256);
     if (e != hipSuccess) {
         (void)hipGetLastError();
@@ -297,7 +305,12 @@ void free_stack(fg_stack* s) {
     }
     recycle_events(s);
     for (hipEvent_t e : s->ev_pool) (void)hipEventDestroy(e);
-    for (void* p : s->allocs) (void)hipFree(p);
+    for (void* p : s->allocs) {
+        if (s->ext.alloc)
+            s->ext.free(s->ext.user, p);
+        else
+            (void)hipFree(p);
+    }
     if (s->own_stream && s->stream) (void)hipStreamDestroy(s->stream);
     delete s;
 }
@@ -697,7 +710,8 @@ size_t fg_plan_stack_bytes(const fg_stack_opts* o, double budget, int* cached) {
     return pl.bytes;
 }
 
-int fg_stack_create_budget(fg_context* ctx, const fg_stack_opts* o, double budget, fg_stack** out) {
+int fg_stack_create_budget(fg_context* ctx, const fg_stack_opts* o, double budget, fg_stack** out,
+                           const fg_dev_allocator* allocator) {
     if (!ctx || !o || !out) return fail(FG_ERR_INVALID_ARG, "null argument");
     *out = nullptr;
     int rc = check_stack_opts(o);
@@ -714,6 +728,7 @@ int fg_stack_create_budget(fg_context* ctx, const fg_stack_opts* o, double budge
     fg_stack* s = new (std::nothrow) fg_stack();
     if (!s) return fail(FG_ERR_OOM_HOST, "stack allocation");
     s->ctx = ctx;
+    if (allocator) s->ext = *allocator;
     s->opts = *o;
     s->W = o->width;
     s->H = o->height;

@@ -8,9 +8,17 @@
 // Device bytes of a stack with these options when `budget` bytes are available; *cached tells
 // whether every frame then stays on the device.  0 for invalid options.
 size_t fg_plan_stack_bytes(const fg_stack_opts* o, double budget, int* cached);
+// Device memory source for a stack's buffers (the server's arena): alloc returns nullptr when it
+// cannot serve the request; free takes back what alloc gave.
+struct fg_dev_allocator {
+    void* (*alloc)(void* user, size_t bytes);
+    void (*free)(void* user, void* p);
+    void* user;
+};
 // fg_stack_create with the caching decision taken against `budget` bytes (< 0: the options'
-// memory fraction of free HBM).
-int fg_stack_create_budget(fg_context* ctx, const fg_stack_opts* o, double budget, fg_stack** out);
+// memory fraction of free HBM) and its buffers from `allocator` (nullptr: hipMalloc).
+int fg_stack_create_budget(fg_context* ctx, const fg_stack_opts* o, double budget, fg_stack** out,
+                           const fg_dev_allocator* allocator = nullptr);
 // HIP device of a context.
 int fg_context_device(const fg_context* ctx);
 // Sets the calling thread's fg_last_error() message and returns `code`.

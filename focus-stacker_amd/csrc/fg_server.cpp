@@ -6,9 +6,11 @@
 // device, so each result equals the request run alone.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <chrono>
 #include <condition_variable>
 #include <deque>
+#include <map>
 #include <memory>
 #include <mutex>
 #include <string>
@@ -50,6 +52,83 @@ struct Pooled {
     uint64_t last_use;
 };
 
+// Device memory of a server: chunks from hipMalloc, added on demand up to the budget and kept
+// until the server ends, so serving itself never calls hipMalloc or hipFree (hipFree waits for
+// the whole device); stacks take 256-byte aligned ranges, first fit, with coalescing frees.
+struct Arena {
+    struct Chunk {
+        char* base;
+        size_t size;
+        std::map<size_t, size_t> free;  // offset -> length
+    };
+    std::vector<Chunk> chunks;
+    std::unordered_map<void*, std::pair<size_t, size_t>> live;  // pointer -> (chunk, length)
+    size_t total = 0, limit = 0;
+    bool overflow = false;  // a request running alone may grow the arena past the limit
+    static constexpr size_t kMinChunk = (size_t)8 << 30;
+
+    void* take(size_t c, size_t bytes) {
+        auto& fl = chunks[c].free;
+        for (auto it = fl.begin(); it != fl.end(); ++it) {
+            if (it->second < bytes) continue;
+            const size_t off = it->first, rem = it->second - bytes;
+            fl.erase(it);
+            if (rem) fl[off + bytes] = rem;
+            void* p = chunks[c].base + off;
+            live[p] = {c, bytes};
+            return p;
+        }
+        return nullptr;
+    }
+    void* alloc(size_t bytes) {
+        bytes = (bytes + 255) & ~(size_t)255;
+        for (size_t c = 0; c < chunks.size(); c++)
+            if (void* p = take(c, bytes)) return p;
+        size_t sz = std::max(bytes, kMinChunk);
+        if (total + sz > limit) sz = bytes;
+        if (total + sz > limit && !overflow) return nullptr;
+        void* base = nullptr;
+        if (hipMalloc(&base, sz) != hipSuccess) {
+            (void)hipGetLastError();
+            return nullptr;
+        }
+        chunks.push_back(Chunk{(char*)base, sz, {}});
+        chunks.back().free[0] = sz;
+        total += sz;
+        return take(chunks.size() - 1, bytes);
+    }
+    void release(void* p) {
+        auto it = live.find(p);
+        if (it == live.end()) return;
+        const size_t c = it->second.first, len = it->second.second;
+        live.erase(it);
+        auto& fl = chunks[c].free;
+        size_t off = (size_t)((char*)p - chunks[c].base), n = len;
+        auto nx = fl.lower_bound(off);
+        if (nx != fl.end() && off + n == nx->first) {  // merge with the following range
+            n += nx->second;
+            nx = fl.erase(nx);
+        }
+        if (nx != fl.begin()) {  // and with the preceding one
+            auto pv = std::prev(nx);
+            if (pv->first + pv->second == off) {
+                pv->second += n;
+                return;
+            }
+        }
+        fl[off] = n;
+    }
+    void destroy() {
+        for (Chunk& c : chunks) (void)hipFree(c.base);
+        chunks.clear();
+        live.clear();
+        total = 0;
+    }
+};
+
+void* arena_alloc(void* u, size_t bytes) { return ((Arena*)u)->alloc(bytes); }
+void arena_free(void* u, void* p) { ((Arena*)u)->release(p); }
+
 bool same_geometry(const fg_stack_opts& a, const fg_stack_opts& b) {
     return a.method == b.method && a.contrast_window_size == b.contrast_window_size &&
            a.hybrid_sharpness == b.hybrid_sharpness && a.frame_count == b.frame_count && a.width == b.width &&
@@ -76,8 +155,11 @@ struct fg_server {
     std::condition_variable cv_work, cv_done;
     std::thread worker;
     bool stop = false;
+    Arena arena;
+    fg_dev_allocator hook{arena_alloc, arena_free, nullptr};
 
     void run();
+    void drop_idle();
     bool admit(std::unique_lock<std::mutex>& lk);
     void start(int slot, const std::shared_ptr<Request>& r);
     void retire(int slot, int state);
@@ -113,7 +195,27 @@ void fg_server::start(int slot, const std::shared_ptr<Request>& r) {
         o.stream = streams[slot];
         o.debug_frame = -1;
         o.memory_fraction = 0.0f;
-        rc = fg_stack_create_budget(ctx, &o, budget > in_use ? budget - in_use : 0.0, &st);
+        const double left = budget > in_use ? budget - in_use : 0.0;
+        rc = fg_stack_create_budget(ctx, &o, left, &st, &hook);
+        if (rc == FG_ERR_OOM_DEVICE) {  // a fragmented arena: give back the idle stacks and retry
+            drop_idle();
+            rc = fg_stack_create_budget(ctx, &o, left, &st, &hook);
+        }
+        if (rc == FG_ERR_OOM_DEVICE) {
+            bool others = false;
+            for (int i = 0; i < nslots; i++) others = others || (i != slot && running[i]);
+            if (others) {  // wait for running stacks to return their memory
+                r->state = FG_REQ_QUEUED;
+                running[slot] = nullptr;
+                queue.push_front(r);
+                return;
+            }
+            // alone: a stack larger than the whole budget still runs (uncached, as the
+            // reference does when its frame cache does not fit)
+            arena.overflow = true;
+            rc = fg_stack_create_budget(ctx, &o, left, &st, &hook);
+            arena.overflow = false;
+        }
         if (rc == FG_OK) in_use += (double)r->bytes;
     }
     r->stack = st;
@@ -130,6 +232,17 @@ void fg_server::start(int slot, const std::shared_ptr<Request>& r) {
         r->error = fg_last_error();
         (void)hipStreamSynchronize(streams[slot]);
         retire(slot, rc);
+    }
+}
+
+// Destroys every pooled stack (lock held).
+void fg_server::drop_idle() {
+    for (auto& pool : idle) {
+        for (Pooled& p : pool) {
+            fg_stack_destroy(p.stack);
+            in_use -= (double)p.bytes;
+        }
+        pool.clear();
     }
 }
 
@@ -203,6 +316,7 @@ bool fg_server::admit(std::unique_lock<std::mutex>&) {
         }
         queue.pop_front();
         start(slot, r);
+        if (r->state == FG_REQ_QUEUED) break;  // requeued: memory comes back as others finish
         any = true;
     }
     return any;
@@ -253,6 +367,8 @@ int fg_server_create(fg_context* ctx, const fg_server_opts* opts, fg_server** ou
     (void)hipMemGetInfo(&free_b, &total_b);
     const float frac = opts && opts->memory_fraction > 0.0f ? opts->memory_fraction : 0.0f;
     s->budget = frac > 0.0f ? (double)frac * (double)free_b : (double)free_b - 1.0e9;
+    s->arena.limit = s->budget > 0.0 ? (size_t)s->budget : 0;
+    s->hook.user = &s->arena;
     s->streams.resize(s->nslots, nullptr);
     s->done_ev.resize(s->nslots, nullptr);
     s->running.resize(s->nslots);
@@ -279,6 +395,7 @@ void fg_server_destroy(fg_server* s) {
     if (s->worker.joinable()) s->worker.join();
     for (auto& pool : s->idle)
         for (Pooled& p : pool) fg_stack_destroy(p.stack);
+    s->arena.destroy();
     for (hipEvent_t e : s->done_ev)
         if (e) (void)hipEventDestroy(e);
     for (hipStream_t st : s->streams)
