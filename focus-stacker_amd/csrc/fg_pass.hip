@@ -90,21 +90,21 @@ int pass1_rows(int window) { return window / 2 <= 4 ? 32 : 64; }
 // colour for the hybrid method), window-column sums, the tile's running norm sum and union
 // support and, for the hybrid method, its winning weight and colour
 struct Pass1Lds {
-    size_t gray, alpha, colour, csum, csq, ccnt, tS, tU, tB, tH, total;
+    size_t gray, alpha, colour, stage, tS, tU, tB, tH, total;
 };
+// staged source box: 4 waves x StageGeom::NR rows of 64 x StageGeom::NC dwords
+constexpr size_t kStageBytesU8 = 4 * 12 * 64 * 1 * 4, kStageBytesU16 = 4 * 10 * 64 * 2 * 4;
 __host__ __device__ inline Pass1Lds pass1_lds(int rh, int th, int sb, bool hybrid) {
     Pass1Lds o;
     const size_t reg = (size_t)rh * P1C, til = (size_t)th * P1C, pt = sb == 1 ? 4 : 8;
     size_t at = 0;
-    o.csum = at;   at += til * 8;
-    o.csq = at;    at += til * 8;
+    o.stage = at;  at += sb == 1 ? kStageBytesU8 : kStageBytesU16;
     o.gray = at;   at += reg * 4;
     o.tS = at;     at += til * 4;
     o.tB = at;     at += hybrid ? til * 4 : 0;
     o.colour = at; at += hybrid ? reg * pt : 0;
     o.tH = at;     at += hybrid ? til * pt : 0;
     o.alpha = at;  at += reg * 2;
-    o.ccnt = at;   at += til;
     o.tU = at;     at += til;
     o.total = (at + 15) & ~(size_t)15;
     return o;
@@ -211,6 +211,9 @@ template <>
 struct StageGeom<uint16_t> {
     static constexpr int NR = 10, NC = 2;
 };
+
+static_assert(kStageBytesU8 == 4 * StageGeom<uint8_t>::NR * 64 * StageGeom<uint8_t>::NC * 4, "staging size");
+static_assert(kStageBytesU16 == 4 * StageGeom<uint16_t>::NR * 64 * StageGeom<uint16_t>::NC * 4, "staging size");
 
 struct StageBox {
     int ok;          // frame staged
@@ -364,8 +367,143 @@ __device__ __forceinline__ void gather_staged(const FrameDesc& f, const StageBox
     g.ok = true;
 }
 
+// A frame's fast-path constants, copied out of its descriptor once per frame (the descriptor sits
+// in global memory, where the kernel's own stores could alias it for the compiler).
+struct FastFrame {
+    double b1, b4, b7;           // the homography's column for the pano y coordinate
+    double dcy;
+    double focal, csx, csy, scx, scy;
+    int roi_top, legacy;
+    uint32_t sw;
+    double wlim, hlim;           // interior bounds: x < sw - 1, y < sh - 1
+};
+
+__device__ __forceinline__ FastFrame fast_frame(const FrameDesc& f) {
+    const RemapParams& p = f.p;
+    return FastFrame{p.B[1], p.B[4], p.B[7], p.dcy, p.focal, p.csx, p.csy, p.scx, p.scy, p.roi_top, p.legacy,
+                     f.sw, (double)f.sw - 1.0, (double)f.sh - 1.0};
+}
+
+// The interior arithmetic of samplePixelBilinear on taps t[12] (rows y0, y0+1 x columns x0, x0+1
+// x rgb) with the reference's f64 weights; false within `band` of a rounding tie.
+__device__ __forceinline__ bool bilinear_taps(const uint32_t t[12], double fx, double fy, double band, uint32_t o[3]) {
+    const double omx = 1.0 - fx, omy = 1.0 - fy;
+    const double w00 = omx * omy, w10 = fx * omy, w01 = omx * fy, w11 = fx * fy;
+    const double sup = ((w00 + w10) + w01) + w11;
+    const double inv = rcp_f64(sup);
+    bool ok = true;
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        const double v = (((double)t[c] * w00 + (double)t[3 + c] * w10) + (double)t[6 + c] * w01) + (double)t[9 + c] * w11;
+        const double q = v * inv + 0.5;
+        const double fl = floor(q);
+        const double d = q - fl;
+        ok = ok && d > band && d < 1.0 - band;
+        o[c] = (uint32_t)fl;
+    }
+    return ok;
+}
+
+// Region pixel in row gy of the homography fast path: coordinate by one reciprocal
+// (fast_coord), interior taps from the staged box (or from global memory when the frame is not
+// staged).  False where the exact path has to decide.
+template <typename T>
+__device__ __forceinline__ bool remap_fast(const FastFrame& ff, const ColHomog& ch, const StageBox& s,
+                                           const uint8_t* stage, uint32_t lead, const T* __restrict__ src, int gy,
+                                           double band, uint32_t o[3]) {
+    const double pyd = (double)(ff.roi_top + gy) - ff.dcy;
+    const double lx = ch.cx + ff.b1 * pyd;
+    const double ly = ch.cy + ff.b4 * pyd;
+    const double denom = -(ch.cz + ff.b7 * pyd);
+    if (!(ff.legacy ? denom > 1e-6 : fabs(denom) > 1e-6)) return false;
+    const double inv = rcp_f64(denom);
+    const double x = ff.scx + (ff.focal * (lx * inv) + ff.csx);
+    const double y = ff.scy + (ff.focal * (ly * inv) + ff.csy);
+    if (!(x >= 0.0 && y >= 0.0 && x < ff.wlim && y < ff.hlim)) return false;
+    const double fx0 = floor(x), fy0 = floor(y);
+    const int x0 = (int)fx0, y0 = (int)fy0;
+    uint32_t t[12];
+    if (s.ok) {
+        if (x0 < s.x_lo || y0 < s.y_lo || y0 + 1 >= s.y_lo + s.rows) return false;
+        const uint32_t xb = (uint32_t)(x0 - s.x_lo) * 3u * (uint32_t)sizeof(T);
+        if (xb + 6u * (uint32_t)sizeof(T) + 3u > 4u * (uint32_t)s.dw) return false;
+        const uint32_t rowb = ff.sw * 3u * (uint32_t)sizeof(T);
+        const uint32_t g0 = lead + ((uint32_t)y0 * ff.sw + (uint32_t)s.x_lo) * 3u * (uint32_t)sizeof(T);
+        const uint32_t r0 = (uint32_t)(y0 - s.y_lo);
+        const uint32_t a0 = r0 * 4u * (uint32_t)s.dw + (g0 & 3u) + xb;
+        const uint32_t a1 = (r0 + 1u) * 4u * (uint32_t)s.dw + ((g0 + rowb) & 3u) + xb;
+#pragma unroll
+        for (int c = 0; c < 6; c++) {
+            if (sizeof(T) == 1) {
+                t[c] = stage[a0 + c];
+                t[6 + c] = stage[a1 + c];
+            } else {
+                t[c] = *(const uint16_t*)&stage[a0 + 2 * c];
+                t[6 + c] = *(const uint16_t*)&stage[a1 + 2 * c];
+            }
+        }
+    } else {
+        const __attribute__((address_space(1))) T* px = as_global(src);
+        const uint32_t rs = ff.sw * 3u;
+        const uint32_t b00 = (uint32_t)y0 * rs + (uint32_t)x0 * 3u;
+#pragma unroll
+        for (int c = 0; c < 6; c++) {
+            t[c] = px[b00 + c];
+            t[6 + c] = px[b00 + rs + c];
+        }
+    }
+    return bilinear_taps(t, x - fx0, y - fy0, band, o);
+}
+
+// Cross-lane moves within a wave (DPP wave_shr:1 / wave_shl:1): the value of lane - 1 / lane + 1,
+// 0 at the edge lane.  Every lane of the wave must execute them.
+__device__ __forceinline__ int lane_prev(int v) { return __builtin_amdgcn_update_dpp(0, v, 0x138, 0xf, 0xf, false); }
+__device__ __forceinline__ int lane_next(int v) { return __builtin_amdgcn_update_dpp(0, v, 0x130, 0xf, 0xf, false); }
+__device__ __forceinline__ double lane_prev(double v) {
+    return __hiloint2double(lane_prev(__double2hiint(v)), lane_prev(__double2loint(v)));
+}
+__device__ __forceinline__ double lane_next(double v) {
+    return __hiloint2double(lane_next(__double2hiint(v)), lane_next(__double2loint(v)));
+}
+
+// Window sum over lanes c-b .. c+b, added in lane order (the order of the reference's column
+// sums, computeRange src/fuse/contrast.zig:218-281).
+template <int B, typename V>
+__device__ __forceinline__ V lane_window_sum_b(V x) {  // the 2B neighbours kept: 2B moves
+    V m[B], p[B];
+    m[0] = lane_prev(x);
+    p[0] = lane_next(x);
+#pragma unroll
+    for (int i = 1; i < B; i++) {
+        m[i] = lane_prev(m[i - 1]);
+        p[i] = lane_next(p[i - 1]);
+    }
+    V s = m[B - 1];
+#pragma unroll
+    for (int i = B - 2; i >= 0; i--) s = s + m[i];
+    s = s + x;
+#pragma unroll
+    for (int i = 0; i < B; i++) s = s + p[i];
+    return s;
+}
+
+template <typename V>
+__device__ __forceinline__ V lane_window_sum(V x, int b) {
+    if (b == 2) return lane_window_sum_b<2>(x);
+    if (b == 1) return lane_window_sum_b<1>(x);
+    if (b == 3) return lane_window_sum_b<3>(x);
+    // wider windows: forward sums x(c) + ... + x(c+2b), then moved back by b lanes
+    V s = x, y = x;
+    for (int i = 0; i < 2 * b; i++) {
+        y = lane_next(y);
+        s = s + y;
+    }
+    for (int i = 0; i < b; i++) s = lane_prev(s);
+    return s;
+}
+
 template <typename T, int RH>
-__global__ void __launch_bounds__(256) k_pass1(Pass1Args a) {
+__global__ void __launch_bounds__(256, sizeof(T) == 1 ? 4 : 3) k_pass1(Pass1Args a) {
     using P = Pack<T>;
     using PT = typename P::type;
     using G = StageGeom<T>;
@@ -379,10 +517,7 @@ __global__ void __launch_bounds__(256) k_pass1(Pass1Args a) {
     float* gray = (float*)(smem + L.gray);            // [RH][64] gray, -1 where alpha == 0 or outside
     uint16_t* alph = (uint16_t*)(smem + L.alpha);     // [RH][64] alpha
     PT* colour = (PT*)(smem + L.colour);              // [RH][64] hybrid: packed rgb
-    double* csum = (double*)(smem + L.csum);          // [th][64] window-column sums of gray
-    double* csq = (double*)(smem + L.csq);            // [th][64] ... of the f32 square
-    uint32_t* stage = (uint32_t*)(smem + L.csum);     // staged source box, over csum/csq
-    uint8_t* ccnt = smem + L.ccnt;                    // [th][64] ... of supported samples
+    uint32_t* stage = (uint32_t*)(smem + L.stage);    // staged source box of the current frame
     float* tS = (float*)(smem + L.tS);                // [th][64] norm sum of the tile
     uint8_t* tU = smem + L.tU;                        // [th][64] union support
     float* tB = (float*)(smem + L.tB);                // [th][64] hybrid: best weight
@@ -469,7 +604,9 @@ __global__ void __launch_bounds__(256) k_pass1(Pass1Args a) {
         float* wout = keep_w ? a.weights + slot * fpx : nullptr;
         const bool fast = f.kind == 0 && f.p.mode == REMAP_HOMOGRAPHY;
         const uint32_t lead = (uint32_t)((uintptr_t)f.src & 3u);
-        const double wlim = (double)f.sw - 1.0, hlim = (double)f.sh - 1.0;
+        const FastFrame ff = fast_frame(f);
+        const T* fsrc = (const T*)f.src;
+        const FrameDesc* fptr = a.frames + k;
         const ColHomog chm = fast ? col_homog(f.p, gx) : ColHomog{0.0, 0.0, 0.0};
         // remap the region (remapRigidImage); gray (fillAverageFromLoaded) where alpha > 0
 #pragma unroll 1
@@ -477,24 +614,10 @@ __global__ void __launch_bounds__(256) k_pass1(Pass1Args a) {
             const int gy = Y0 - b + r;
             uint32_t o[4] = {0, 0, 0, 0};
             if (gx >= 0 && gy >= 0 && gx < W && gy < H) {
-                bool done = false;
-                if (fast) {
-                    FastTaps t;
-                    t.ok = false;
-                    double sx, sy;
-                    if (fast_coord(f.p, chm, gy, &sx, &sy)) {
-                        if (box.ok)
-                            gather_staged<T>(f, box, (const uint8_t*)stage, lead, sx, sy, wlim, hlim, t);
-                        else if (sx >= 0.0 && sy >= 0.0 && sx < wlim && sy < hlim)
-                            gather_at<T>((const T*)f.src, f.sw, sx, sy, t);
-                    }
-                    if (t.ok && resolve_fast(t, band, o)) {
-                        o[3] = SampleMax<T>::i;
-                        done = true;
-                    }
-                }
-                if (!done) {
-                    const uint64_t e = frame_rgba_exact<T>(f, gx, gy);
+                if (fast && remap_fast<T>(ff, chm, box, (const uint8_t*)stage, lead, fsrc, gy, band, o)) {
+                    o[3] = SampleMax<T>::i;
+                } else {
+                    const uint64_t e = frame_rgba_exact<T>(*fptr, gx, gy);
                     o[0] = (uint32_t)e & 0xffffu;
                     o[1] = (uint32_t)(e >> 16) & 0xffffu;
                     o[2] = (uint32_t)(e >> 32) & 0xffffu;
@@ -524,71 +647,62 @@ __global__ void __launch_bounds__(256) k_pass1(Pass1Args a) {
         StageBox nbox = k + 1 < a.k1 ? stage_box<T>(corners, (int)a.frames[k + 1].sw, (int)a.frames[k + 1].sh)
                                      : StageBox{0, 0, 0, 0, 0};
         if (nbox.ok) stage_load<T>(a.frames[k + 1], nbox, lane, wave, pf);
-        // window-column sums of this wave's output rows, sliding down the column
-        // (computeRange src/fuse/contrast.zig:201-217: f64 sums of x and of the f32 product x*x)
-        if (contrast_ok && r0 < r1) {
-            double s = 0.0, q = 0.0;
-            int n = 0;
-#pragma unroll 1
-            for (int dy = 0; dy <= 2 * b; dy++) {
-                const float v = gray[(r0 + dy) * P1C + lane];
-                if (v >= 0.0f) {
-                    s += (double)v;
-                    q += (double)(v * v);
-                    n++;
-                }
-            }
-            csum[r0 * P1C + lane] = s;
-            csq[r0 * P1C + lane] = q;
-            ccnt[r0 * P1C + lane] = n;
-#pragma unroll 1
-            for (int r = r0 + 1; r < r1; r++) {
-                const float vi = gray[(r + 2 * b) * P1C + lane];
-                const float vo = gray[(r - 1) * P1C + lane];
-                if (vi >= 0.0f) {
-                    s += (double)vi;
-                    q += (double)(vi * vi);
-                    n++;
-                }
-                if (vo >= 0.0f) {
-                    s -= (double)vo;
-                    q -= (double)(vo * vo);
-                    n--;
-                }
-                csum[r * P1C + lane] = s;
-                csq[r * P1C + lane] = q;
-                ccnt[r * P1C + lane] = n;
-            }
-        }
-        lds_barrier();
+        // contrast of this wave's output rows: window-column sums slide down the column in
+        // registers (lane = column), the window's columns are added across lanes
+        // (computeRange src/fuse/contrast.zig:172-282: f64 sums of x and of the f32 product x*x)
+        double vs = 0.0, vq = 0.0;
+        int vn = 0;
 #pragma unroll 1
         for (int r = r0; r < r1; r++) {
+            if (contrast_ok) {
+                if (r == r0) {
+                    for (int dy = 0; dy <= 2 * b; dy++) {
+                        const float v = gray[(r0 + dy) * P1C + lane];
+                        if (v >= 0.0f) {
+                            vs += (double)v;
+                            vq += (double)(v * v);
+                            vn++;
+                        }
+                    }
+                } else {
+                    const float vi = gray[(r + 2 * b) * P1C + lane];
+                    const float vo = gray[(r - 1) * P1C + lane];
+                    if (vi >= 0.0f) {
+                        vs += (double)vi;
+                        vq += (double)(vi * vi);
+                        vn++;
+                    }
+                    if (vo >= 0.0f) {
+                        vs -= (double)vo;
+                        vq -= (double)(vo * vo);
+                        vn--;
+                    }
+                }
+            }
+            double s = 0.0, q = 0.0;
+            int n = 0;
+            if (contrast_ok) {  // every lane takes part in the cross-lane sums
+                s = lane_window_sum(vs, b);
+                q = lane_window_sum(vq, b);
+                n = lane_window_sum(vn, b);
+            }
             const int y = Y0 + r;
             if (!col_ok || y >= H) continue;
             const size_t i = (size_t)y * W + ox;
             float stdw = 0.0f;
-            if (contrast_ok && ox >= b && ox < W - b && y >= b && y < H - b && gray[(r + b) * P1C + lane] >= 0.0f) {
-                double s = 0.0, q = 0.0;
-                int n = 0;
-                for (int dx = 0; dx <= 2 * b; dx++) {
-                    const int ci = r * P1C + lane - b + dx;
-                    s += csum[ci];
-                    q += csq[ci];
-                    n += ccnt[ci];
+            if (contrast_ok && ox >= b && ox < W - b && y >= b && y < H - b && gray[(r + b) * P1C + lane] >= 0.0f &&
+                n > 1) {  // sampleStdDev src/fuse/contrast.zig:284-290
+                double m, var;
+                if (n == nfull) {
+                    m = div_markstein(s * s, nf_full, r_full);
+                    var = div_markstein(q - m, nf_full1, r_full1);
+                } else {
+                    const double nf = (double)n, den = (double)(n - 1);
+                    m = (s * s) / nf;
+                    var = (q - m) / den;
                 }
-                if (n > 1) {  // sampleStdDev src/fuse/contrast.zig:284-290
-                    double m, var;
-                    if (n == nfull) {
-                        m = div_markstein(s * s, nf_full, r_full);
-                        var = div_markstein(q - m, nf_full1, r_full1);
-                    } else {
-                        const double nf = (double)n, den = (double)(n - 1);
-                        m = (s * s) / nf;
-                        var = (q - m) / den;
-                    }
-                    if (!(var > 0.0)) var = 0.0;
-                    stdw = (float)sqrt(var);
-                }
+                if (!(var > 0.0)) var = 0.0;
+                stdw = (float)sqrt(var);
             }
             const uint32_t alpha = alph[(r + b) * P1C + lane];
             const float af = alpha == SampleMax<T>::i
@@ -606,12 +720,11 @@ __global__ void __launch_bounds__(256) k_pass1(Pass1Args a) {
                 tH[t] = colour[(r + b) * P1C + lane];
             }
         }
-        lds_barrier();
+        // the next frame's box: its loads were issued before the contrast, the remap of this
+        // frame is done with the staging (first barrier above)
         box = nbox;
-        if (box.ok) {
-            stage_store<T>(box, lane, wave, pf, stage);
-            lds_barrier();
-        }
+        if (box.ok) stage_store<T>(box, lane, wave, pf, stage);
+        lds_barrier();
     }
 
 #pragma unroll 1
