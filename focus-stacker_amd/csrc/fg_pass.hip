@@ -970,6 +970,14 @@ __global__ void __launch_bounds__(256) k_pass2(Pass2Args a, LevelArgs g) {
         }
     }
 
+    // level index of this thread's region pixels (-1 outside the level), the same for every frame
+    int gidx[NPF];
+#pragma unroll
+    for (int j = 0; j < NPF; j++) {
+        const int idx = threadIdx.x + 256 * j;
+        const int gx = X0 - 4 + idx % RW, gy = Y0 - 4 + idx / RW;
+        gidx[j] = idx < NREG && gx >= 0 && gy >= 0 && gx < g.w && gy < g.h ? gy * g.w + gx : -1;
+    }
     // the next frame's level-l region, in flight while the current frame is reduced and expanded
     PT pf_px[L0 ? NPF : 1];
     float pf_w[L0 ? NPF : 1];
@@ -981,10 +989,8 @@ __global__ void __launch_bounds__(256) k_pass2(Pass2Args a, LevelArgs g) {
         const float4* lv = L0 ? nullptr : a.lev + slot * a.lev_stride + g.lev_off;
 #pragma unroll
         for (int j = 0; j < NPF; j++) {
-            const int idx = threadIdx.x + 256 * j;
-            const int gx = X0 - 4 + idx % RW, gy = Y0 - 4 + idx / RW;
-            if (idx < NREG && gx >= 0 && gy >= 0 && gx < g.w && gy < g.h) {
-                const int i = gy * g.w + gx;
+            const int i = gidx[j];
+            if (i >= 0) {
                 if (L0) {
                     pf_px[j] = rg[i];
                     pf_w[j] = wts[i];
@@ -1006,9 +1012,8 @@ __global__ void __launch_bounds__(256) k_pass2(Pass2Args a, LevelArgs g) {
             for (int j = 0; j < NPF; j++) {
                 const int idx = threadIdx.x + 256 * j;
                 if (idx >= NREG) continue;
-                const int gx = X0 - 4 + idx % RW, gy = Y0 - 4 + idx / RW;
                 float4 v = make_float4(-1.0f, 0.0f, 0.0f, -1.0f);
-                if (gx >= 0 && gy >= 0 && gx < g.w && gy < g.h) {
+                if (gidx[j] >= 0) {
                     if (L0) {
                         // fillRgbBase + fillBinarySupport (src/fuse/pyramid.zig:502-503) and
                         // normalizeWeightsInto (:303-316) with the union support (:202)
