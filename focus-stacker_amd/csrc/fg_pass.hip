@@ -939,6 +939,9 @@ __global__ void __launch_bounds__(256) k_pass2(Pass2Args a, LevelArgs g) {
     const int ke = SPLIT ? kb + 1 : a.k1;
     const bool clamp_regime = (g.w > g.h ? g.w : g.h) > 512;  // exact_expand_max_dim, src/fuse/pyramid.zig:8
     const int ox1 = X0 / 2 - 1, oy1 = Y0 / 2 - 1;
+    // a tile whose expand taps never reach a level edge: the clamp-to-edge expand needs no clamping
+    const bool inner = clamp_regime && !g.top && X0 >= 4 && Y0 >= 4 && 2 * (X0 / 2 + TW / 2 + 2) <= 2 * g.nw - 2 &&
+                       2 * (Y0 / 2 + TH / 2 + 2) <= 2 * g.nh - 2 && X0 + TW <= g.w && Y0 + TH <= g.h;
 
     float ar[TL::PPT], ag[TL::PPT], ab[TL::PPT];
 #pragma unroll
@@ -1122,7 +1125,18 @@ __global__ void __launch_bounds__(256) k_pass2(Pass2Args a, LevelArgs g) {
                 const int r = idx / TW, x = X0 + idx % TW;
                 const int gj = oy1 + r;
                 float h0 = 0.0f, h1 = 0.0f, h2 = 0.0f;
-                if (x < g.w && gj >= 0 && gj < g.nh) {
+                if (inner) {  // horizontalExpandRgbRow without the clamps it cannot need here
+                    const int c1 = r * NW - ox1 + (x >> 1);
+                    if ((x & 1) == 0) {
+                        h0 = ((nR[c1 - 1] + 6.0f * nR[c1]) + nR[c1 + 1]) * 0.125f;
+                        h1 = ((nG[c1 - 1] + 6.0f * nG[c1]) + nG[c1 + 1]) * 0.125f;
+                        h2 = ((nB[c1 - 1] + 6.0f * nB[c1]) + nB[c1 + 1]) * 0.125f;
+                    } else {
+                        h0 = (nR[c1] + nR[c1 + 1]) * 0.5f;
+                        h1 = (nG[c1] + nG[c1 + 1]) * 0.5f;
+                        h2 = (nB[c1] + nB[c1 + 1]) * 0.5f;
+                    }
+                } else if (x < g.w && gj >= 0 && gj < g.nh) {
                     const int i = x >> 1;
                     const int rb = r * NW - ox1;
                     if (clamp_regime) {
@@ -1179,7 +1193,17 @@ __global__ void __launch_bounds__(256) k_pass2(Pass2Args a, LevelArgs g) {
                 const int jj = y >> 1;
                 const int xx = x - X0;
                 f3 e;
-                if (clamp_regime) {
+                if (inner) {
+                    const int q1 = (jj - oy1) * TW + xx;
+                    if ((y & 1) == 0) {
+                        const int q0 = q1 - TW, q2 = q1 + TW;
+                        e = mk3(((hR[q0] + 6.0f * hR[q1]) + hR[q2]) * 0.125f,
+                                ((hG[q0] + 6.0f * hG[q1]) + hG[q2]) * 0.125f,
+                                ((hB[q0] + 6.0f * hB[q1]) + hB[q2]) * 0.125f);
+                    } else {
+                        e = mk3((hR[q1] + hR[q1]) * 0.5f, (hG[q1] + hG[q1]) * 0.5f, (hB[q1] + hB[q1]) * 0.5f);
+                    }
+                } else if (clamp_regime) {
                     if ((y & 1) == 0) {
                         const int q0 = (clamp_coord(jj - 1, g.nh) - oy1) * TW + xx;
                         const int q1 = (clamp_coord(jj, g.nh) - oy1) * TW + xx;
