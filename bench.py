@@ -57,7 +57,7 @@ def pct(v, q):
     return v[k]
 
 
-def cpu_baseline(frames_np, poses, roi, budget_s=30.0):
+def cpu_baseline(frames_np, poses, roi, method=0):
     """The oracle (single-threaded C restatement) on a bounded sample of the same workload:
     frames 0-1 at full size over the top half of the ROI."""
     from oracle import Oracle, make_pose
@@ -65,7 +65,7 @@ def cpu_baseline(frames_np, poses, roi, budget_s=30.0):
     l, t, r, b = roi
     sub = (l, t, r, t + (b - t) // 2)
     t0 = time.perf_counter()
-    o.stack(frames_np[:2], [make_pose(**p) for p in poses[:2]], sub)
+    o.stack(frames_np[:2], [make_pose(**p) for p in poses[:2]], sub, method=method)
     dt = time.perf_counter() - t0
     px = 2 * (sub[2] - sub[0]) * (sub[3] - sub[1])
     return {"value": round(px / dt / 1e6, 3), "unit": "MPix/s", "cores": 1, "kind": "port",
@@ -228,6 +228,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-host", action="store_true", help="skip the host-fed (PCIe-inclusive) step")
     ap.add_argument("--profile-only", action="store_true", help="one profiled step (for rocprofv3 runs)")
+    ap.add_argument("--config", type=int, choices=(1, 2), default=1,
+                    help="1: 30 x 7952x5304 u8 (the reported line); 2: 143 x 7952x5304 u16, three focal planes")
+    ap.add_argument("--method", choices=("pyramid", "hybrid", "hardmask", "softmask"), default="pyramid")
     ap.add_argument("--workload", choices=("stack", "batch", "serve"), default="stack",
                     help="stack: the 30 x 7952x5304 stack (default); batch / serve: uneven stack mixes")
     ap.add_argument("--stacks", type=int, default=64)
@@ -251,13 +254,26 @@ def main():
     import synth
 
     dev = torch.device("cuda:0")
-    spec = synth.config_spec(1)
+    spec = synth.config_spec(args.config)
     if args.frames:
         spec.frames = args.frames
     n = spec.frames
-    print(f"bench: generating {n} x {spec.width}x{spec.height} u8 frames on the GPU", file=sys.stderr, flush=True)
+    u16 = spec.dtype == "u16"
+    st_type = fg.SAMPLE_U16 if u16 else fg.SAMPLE_U8
+    s_b = 2 if u16 else 1
+    tname = "u16" if u16 else "u8"
+    method = {"pyramid": fg.METHOD_PYRAMID, "hybrid": fg.METHOD_HYBRID, "hardmask": fg.METHOD_HARDMASK,
+              "softmask": fg.METHOD_SOFTMASK}[args.method]
+    print(f"bench: generating {n} x {spec.width}x{spec.height} {tname} frames on the GPU", file=sys.stderr, flush=True)
     gen = synth.Stack(spec, device=dev)
-    frames = [gen.frame(k) for k in range(n)]
+    frames = []
+    for k in range(n):
+        f = gen.frame(k)
+        if u16:  # int32-coded samples to 16-bit storage
+            f = (f - (f > 32767).to(torch.int32) * 65536).to(torch.int16)
+        frames.append(f)
+        if (k + 1) % 20 == 0:
+            print(f"bench: {k + 1} frames", file=sys.stderr, flush=True)
     poses = gen.poses
     del gen
     torch.cuda.synchronize()
@@ -265,11 +281,11 @@ def main():
     fposes = [fg.pose(p) for p in poses]
     roi = fg.common_overlap_roi([(spec.width, spec.height)] * n, fposes)
     w, h = roi[2] - roi[0], roi[3] - roi[1]
-    desc = fg.ImageDesc(spec.width, spec.height, 3, 0, fg.SAMPLE_U8)
+    desc = fg.ImageDesc(spec.width, spec.height, 3, 0, st_type)
     ctx = fg.Context(0, memory_fraction=0.8)
-    stack = fg.Stack(ctx, n, w, h, fg.SAMPLE_U8, max_levels=args.levels)
+    stack = fg.Stack(ctx, n, w, h, st_type, method=method, max_levels=args.levels)
     levels = len(stack.levels())
-    out = torch.empty((h, w, 3), dtype=torch.uint8, device=dev)
+    out = torch.empty((h, w, 3), dtype=torch.int16 if u16 else torch.uint8, device=dev)
     ext = torch.cuda.ExternalStream(stack.stream())
 
     def step():
@@ -321,7 +337,6 @@ def main():
     value = frame_px / (ms_per_step / 1000.0) / 1e6
     print(f"bench: {ms_per_step:.2f} ms/step, {value:.1f} MPix/s, phases {ph}", file=sys.stderr, flush=True)
 
-    s_b = 1
     kern = {}
     for name, (phase, per_unit) in kernel_models(s_b).items():
         total_ms, launches = ph[phase]
@@ -335,8 +350,9 @@ def main():
     dom = max(kern, key=lambda k: kern[k]["avg_launch_ms"])
     pipe_bytes = alg_bytes_per_frame_px(s_b, n) * frame_px
     pipe_gbs = pipe_bytes / (ms_per_step / 1000.0) / 1e9
-    cfg_name = f"{n} x {spec.width}x{spec.height} RGB u8"
-    traffic = TRAFFIC.get(dom) if (n, spec.width, spec.height, args.levels) == (30, 7952, 5304, 0) else None
+    cfg_name = f"{n} x {spec.width}x{spec.height} RGB {tname}"
+    traffic = (TRAFFIC.get(dom) if (n, spec.width, spec.height, args.levels, tname, args.method) ==
+               (30, 7952, 5304, 0, "u8", "pyramid") else None)
 
     line = {
         "metric": METRIC,
@@ -349,9 +365,9 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "u8",
+        "dtype": tname,
         "data": "synthetic",
-        "config": {"workload": f"{cfg_name} stack (synthetic, seeded), remap + pyramid-contrast, {levels} levels",
+        "config": {"workload": f"{cfg_name} stack (synthetic, seeded), remap + {args.method}-contrast, {levels} levels",
                    "frames": n, "source": [spec.width, spec.height], "roi": list(roi), "levels": levels,
                    "parallelism": "single GPU"},
         "latency_ms": {"p50": round(pct(lat, 50), 3), "p99": round(pct(lat, 99), 3)},
@@ -390,7 +406,7 @@ def main():
         # the same from page-locked buffers (fg_host_alloc), as a decoder writing into them would
         pinned = []
         for f in frames:
-            pa = fg.PinnedArray(tuple(f.shape), np.uint8)
+            pa = fg.PinnedArray(tuple(f.shape), np.uint16 if u16 else np.uint8)
             pa.array[...] = f.cpu().numpy()
             pinned.append(pa)
         t0 = time.perf_counter()
@@ -410,8 +426,9 @@ def main():
         del frames[2:]
         torch.cuda.synchronize()
         print("bench: timing the CPU baseline (oracle) on a bounded sample", file=sys.stderr, flush=True)
-        frames_np = [f.cpu().numpy() for f in frames[:2]]
-        line["cpu_baseline"] = cpu_baseline(frames_np, poses, roi)
+        frames_np = [f.cpu().numpy().view(np.uint16) if u16 else f.cpu().numpy() for f in frames[:2]]
+        omethod = {"pyramid": 0, "hybrid": 1, "hardmask": 2, "softmask": 3}[args.method]
+        line["cpu_baseline"] = cpu_baseline(frames_np, poses, roi, omethod)
     print(json.dumps(line), flush=True)
 
 
