@@ -152,6 +152,9 @@ def lib() -> C.CDLL:
     L.fg_host_alloc.restype = vp
     L.fg_host_free.argtypes = [vp]
     L.fg_host_free.restype = None
+    L.fg_write_tiff.argtypes = [C.c_char_p, C.POINTER(ImageDesc), vp]
+    L.fg_write_aligned_tiff.argtypes = [vp, C.POINTER(ImageDesc), vp, C.POINTER(Pose), C.POINTER(Rect),
+                                        C.c_char_p, u32]
     _lib = L
     return L
 
@@ -232,11 +235,25 @@ class Context:
                              C.byref(rr) if rr is not None else None, _p(out)))
         return out
 
+    def write_aligned_tiff(self, src: np.ndarray, p: Pose, roi, prefix: str, index: int) -> str:
+        """writeAlignedImageTask: remap `src` into `roi` and write "{prefix}_{index:04}.tif"; returns the path."""
+        src = np.ascontiguousarray(src)
+        rr = rect(roi)
+        check(lib().fg_write_aligned_tiff(self.h, C.byref(desc_for(src)), _p(src), C.byref(p),
+                                          C.byref(rr) if rr is not None else None, os.fsencode(prefix), index))
+        return f"{prefix}_{index:04d}.tif"
+
     def remap_device(self, d_src: int, d: ImageDesc, p: Pose, roi, d_dst: int, stream: int | None = None):
         rr = rect(roi)
         check(lib().fg_remap_device(self.h, C.byref(d), C.c_void_p(d_src), C.byref(p),
                                     C.byref(rr) if rr is not None else None, C.c_void_p(d_dst),
                                     C.c_void_p(stream) if stream else None))
+
+
+def write_tiff(path: str, img: np.ndarray, extra: int = 0):
+    """image_io.writeTiff: img is (h, w) or (h, w, channels) u8/u16; the last `extra` channels are alpha."""
+    img = np.ascontiguousarray(img)
+    check(lib().fg_write_tiff(os.fsencode(path), C.byref(desc_for(img, extra)), _p(img)))
 
 
 def common_overlap_roi(sizes, poses):

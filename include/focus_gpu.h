@@ -139,6 +139,15 @@ int fg_remap_device(fg_context* ctx, const fg_image_desc* src, const void* d_src
 int fg_common_overlap_roi(uint32_t n, const uint8_t* active, const uint32_t* widths, const uint32_t* heights,
                           const fg_pose* poses, fg_rect* out);
 
+/* image_io.writeTiff (src/image_io.zig:175-200) with the tags of openTiffWriter (:100-143):
+ * uncompressed, contiguous, 150 dpi, unassociated alpha for the extra channel.  The file is
+ * byte-identical to what that libtiff call sequence writes.  Host only (no GPU work). */
+int fg_write_tiff(const char* path, const fg_image_desc* desc, const void* px);
+/* writeAlignedImageTask (src/remap.zig:134-164) for a decoded frame: fg_remap into `roi`
+ * (NULL = full frame), then fg_write_tiff to "{prefix}_{output_index:04}.tif". */
+int fg_write_aligned_tiff(fg_context* ctx, const fg_image_desc* src, const void* src_px, const fg_pose* pose,
+                          const fg_rect* roi, const char* prefix, uint32_t output_index);
+
 /* Pyramid-contrast stack: the two-pass loop of runPyramidStackFusion
  * (src/stack/pipeline.zig:361-605) / runPyramidPass (src/fuse/pipeline.zig:254-445). */
 int fg_stack_create(fg_context* ctx, const fg_stack_opts* opts, fg_stack** out);

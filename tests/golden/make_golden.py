@@ -9,13 +9,20 @@ s003_identity.npz
     remap  : port3_0000.tif (RGBA u8), the reference's aligned output of that
              frame.  Frame 0 carries the identity pose (src/optimize.zig:139-141),
              so remapping `src` under the identity pose must give `remap`.
+s003_port3_0000_tiff.json
+    the file signature of port3_0000.tif as the reference's writer laid it out
+    (writeTiff / openTiffWriter through libtiff): size, sha256 and the bytes around
+    the pixel data (header, directory, out-of-line tag values).  Writing `remap`
+    above must reproduce the file exactly.
 s003_pair_pose.json
     the solved pose of 0002.jpg from port_pair_m.pto (y/p/r in degrees -> radians,
     v -> hfov_delta, src/parity_pto.zig:140-158), a real solved pose for remap tests.
 """
+import hashlib
 import json
 import os
 import re
+import struct
 
 import numpy as np
 from PIL import Image
@@ -29,6 +36,14 @@ def main():
     tif = np.asarray(Image.open(os.path.join(REF, "port3_0000.tif")))
     assert tif.shape[2] == 4, tif.shape
     np.savez_compressed(os.path.join(HERE, "s003_identity.npz"), src=src, remap=tif)
+
+    raw = open(os.path.join(REF, "port3_0000.tif"), "rb").read()
+    ifd = struct.unpack("<I", raw[4:8])[0]
+    assert raw[8:8 + tif.nbytes] == tif.tobytes()  # one strip of contiguous RGBA at offset 8
+    with open(os.path.join(HERE, "s003_port3_0000_tiff.json"), "w") as f:
+        json.dump({"size": len(raw), "sha256": hashlib.sha256(raw).hexdigest(), "header": raw[:8].hex(),
+                   "pixel_bytes": tif.nbytes, "directory_offset": ifd, "after_pixels": raw[8 + tif.nbytes:].hex()},
+                  f, indent=1)
 
     pano_hfov = None
     poses = []
