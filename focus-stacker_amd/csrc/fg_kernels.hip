@@ -202,7 +202,8 @@ __global__ void k_reduce_clamped(const float* __restrict__ src, int sw, int sh, 
     }
 }
 
-// out[l] = lap_soft + (lap_hard - lap_soft) * mix for l in [hs, he), lap_soft otherwise; top = gaussian soft.
+// out[l] = lap_soft + (lap_hard - lap_soft) * mix for l in [hs, he), lap_soft otherwise
+// (mergeHybridImages src/fuse/pyramid.zig:271-301); the top level's laplacian is the gaussian.
 __global__ void k_lap_mix(const float* __restrict__ gs, const float* __restrict__ gsn, const float* __restrict__ gh,
                           const float* __restrict__ ghn, int w, int h, int nw, int nh, int top, int mixed, float mix,
                           float* __restrict__ out) {
@@ -211,18 +212,17 @@ __global__ void k_lap_mix(const float* __restrict__ gs, const float* __restrict_
     if (i >= n) return;
     const int x = (int)(i % w), y = (int)(i / w);
     f3 s = mk3(gs[i], gs[i + n], gs[i + 2 * n]);
-    if (top) {
-        out[i] = s.x;
-        out[i + n] = s.y;
-        out[i + 2 * n] = s.z;
-        return;
+    f3 hv = mk3(0.0f, 0.0f, 0.0f);
+    if (mixed) hv = mk3(gh[i], gh[i + n], gh[i + 2 * n]);
+    // the top level is the gaussian itself (buildImageLaplacianPyramid src/fuse/pyramid.zig:477-482);
+    // it is mixed like the others when it falls in the hard range, i.e. when the pyramid has at
+    // most start + count levels
+    f3 ls = s;
+    if (!top) {
+        ls = sub3(s, expand_at(PlanarAcc{gsn, (size_t)nw * nh, nw}, nw, nh, w, h, x, y));
+        if (mixed) hv = sub3(hv, expand_at(PlanarAcc{ghn, (size_t)nw * nh, nw}, nw, nh, w, h, x, y));
     }
-    f3 ls = sub3(s, expand_at(PlanarAcc{gsn, (size_t)nw * nh, nw}, nw, nh, w, h, x, y));
-    if (mixed) {
-        f3 hv = mk3(gh[i], gh[i + n], gh[i + 2 * n]);
-        f3 lh = sub3(hv, expand_at(PlanarAcc{ghn, (size_t)nw * nh, nw}, nw, nh, w, h, x, y));
-        ls = add3(ls, mul3r(sub3(lh, ls), mix));
-    }
+    if (mixed) ls = add3(ls, mul3r(sub3(hv, ls), mix));  // soft + (hard - soft) * mix
     out[i] = ls.x;
     out[i + n] = ls.y;
     out[i + 2 * n] = ls.z;

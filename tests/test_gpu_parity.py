@@ -256,6 +256,52 @@ def test_aligned_frames(ctx, fg, oracle, alpha):
     _img_close(out, oracle.fuse_frames(frames, extra=extra))
 
 
+TINY = [(1, 1), (2, 1), (1, 2), (2, 2), (3, 3), (4, 3), (5, 5), (6, 2), (1, 9), (9, 1), (17, 3)]
+TINY_METHODS = ["pyramid", "hybrid", "hardmask", "softmask"]
+
+
+@pytest.mark.parametrize("method", TINY_METHODS)
+@pytest.mark.parametrize("size", TINY, ids=lambda s: f"{s[0]}x{s[1]}")
+def test_tiny_images(ctx, fg, oracle, size, method):
+    """Images smaller than the contrast window (every weight 0, src/fuse/contrast.zig:113-115),
+    one-pixel rows and columns, and a 1x1 image whose level 0 is its top level."""
+    import oracle as om
+    w, h = size
+    rng = np.random.default_rng(w * 100 + h)
+    frames = [rng.integers(0, 256, (h, w, 3)).astype(np.uint8) for _ in range(3)]
+    fm = {"pyramid": fg.METHOD_PYRAMID, "hybrid": fg.METHOD_HYBRID, "hardmask": fg.METHOD_HARDMASK,
+          "softmask": fg.METHOD_SOFTMASK}[method]
+    o_m = {"pyramid": om.METHOD_PYRAMID, "hybrid": om.METHOD_HYBRID, "hardmask": om.METHOD_HARDMASK,
+           "softmask": om.METHOD_SOFTMASK}[method]
+    s = fg.Stack(ctx, len(frames), w, h, method=fm)
+    for f in frames:
+        s.add_frame(f, None, None)
+    out = s.finish()
+    s.close()
+    ref = oracle.fuse_frames(frames, extra=0, method=o_m)
+    if method == "hardmask":
+        assert np.array_equal(out, ref)
+    else:
+        _img_close(out, ref)
+
+
+@pytest.mark.parametrize("dtype", [np.uint8, np.uint16])
+@pytest.mark.parametrize("size", TINY, ids=lambda s: f"{s[0]}x{s[1]}")
+def test_tiny_sources_remapped(ctx, fg, oracle, size, dtype):
+    """The remap path on tiny sources: the identity pose over the full frame, whose border
+    samples have partial support."""
+    from oracle import make_pose
+    w, h = size
+    rng = np.random.default_rng(w * 100 + h + 7)
+    hi = 256 if dtype == np.uint8 else 65536
+    frames = [rng.integers(0, hi, (h, w, 3)).astype(dtype) for _ in range(3)]
+    poses = [dict(), dict(roll=0.01), dict(yaw=-0.004, hfov_delta=0.3)]
+    roi = (0, 0, w, h)
+    s, out = _run_stack(fg, ctx, frames, poses, roi)
+    s.close()
+    _img_close(out, oracle.stack(frames, [make_pose(**p) for p in poses], roi))
+
+
 def test_levels_cap(ctx, fg, oracle):
     spec, frames, poses = _synthetic(dict(frames=3, width=128, height=96))
     from oracle import make_pose
@@ -264,6 +310,20 @@ def test_levels_cap(ctx, fg, oracle):
     s, out = _run_stack(fg, ctx, frames, poses, roi, max_levels=4)
     assert len(s.levels()) == 4
     _img_close(out, oracle.stack(frames, opo, roi, max_levels=4))
+    s.close()
+
+
+@pytest.mark.parametrize("levels", [3, 4, 6, 7])
+def test_hybrid_levels_cap(ctx, fg, oracle, levels):
+    """Hybrid with the pyramid capped: at 6 levels or fewer the hard range [2, 6) reaches the top
+    level (the gaussian), which mergeHybridImages mixes like any other (src/fuse/pyramid.zig:288-298)."""
+    spec, frames, poses = _synthetic(dict(frames=3, width=128, height=96))
+    from oracle import make_pose, METHOD_HYBRID
+    opo = [make_pose(**p) for p in poses]
+    roi = oracle.common_overlap_roi([(spec.width, spec.height)] * spec.frames, opo)
+    s, out = _run_stack(fg, ctx, frames, poses, roi, max_levels=levels, method=fg.METHOD_HYBRID)
+    assert len(s.levels()) == levels
+    _img_close(out, oracle.stack(frames, opo, roi, max_levels=levels, method=METHOD_HYBRID))
     s.close()
 
 
