@@ -17,6 +17,8 @@
 // Every f32 value is rounded where the reference rounds and sums keep its order (frame order
 // for the norm sum and the accumulators, ky-then-kx order in the reduce), so results match the
 // reference's single-threaded arithmetic; see fg_device.h for the expand and remap formulas.
+#include <type_traits>
+
 #include "fg_device.h"
 
 namespace fg {
@@ -404,10 +406,34 @@ __device__ __forceinline__ bool bilinear_taps(const uint32_t t[12], double fx, d
     return ok;
 }
 
+// u8 interior taps in f32, the lerp form on exact integer differences: the value + 0.5 is within
+// 1e-4 of the exact one (weight rounding 2 x 255 x 2^-25, four f32 roundings of at most 2^-17 each
+// at magnitudes below 256, and their propagation).  Outside a band of 2.5e-4 around a rounding tie
+// the exact value, and so the reference's f64 value, rounds the same way; inside it the pixel is
+// redone in f64 by bilinear_taps, whose own band then sends true near-ties to the exact path.
+__device__ __forceinline__ bool bilinear_taps_u8(const uint32_t t[12], double fx, double fy, double band,
+                                                 uint32_t o[3]) {
+    const float fx32 = (float)fx, fy32 = (float)fy;
+    bool near = false;
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        const float p00 = (float)t[c], p10 = (float)t[3 + c], p01 = (float)t[6 + c], p11 = (float)t[9 + c];
+        const float top = fmaf(fx32, p10 - p00, p00);
+        const float bot = fmaf(fx32, p11 - p01, p01);
+        const float q = fmaf(fy32, bot - top, top) + 0.5f;
+        const float fl = floorf(q);
+        near = near || fabsf((q - fl) - 0.5f) > 0.5f - 2.5e-4f;
+        o[c] = (uint32_t)fl;
+    }
+    return near ? bilinear_taps(t, fx, fy, band, o) : true;
+}
+
 // Region pixel in row gy of the homography fast path: coordinate by one reciprocal
 // (fast_coord), interior taps from the staged box (or from global memory when the frame is not
-// staged).  False where the exact path has to decide.
-template <typename T>
+// staged).  False where the exact path has to decide.  STAGED is the block-uniform choice between
+// the two tap sources: a loop that only reads the staged box holds no global loads, so no
+// iteration waits on the vector-memory counter, which also counts the loop's own RGBA stores.
+template <typename T, bool STAGED>
 __device__ __forceinline__ bool remap_fast(const FastFrame& ff, const ColHomog& ch, const StageBox& s,
                                            const uint8_t* stage, uint32_t lead, const T* __restrict__ src, int gy,
                                            double band, uint32_t o[3]) {
@@ -423,7 +449,7 @@ __device__ __forceinline__ bool remap_fast(const FastFrame& ff, const ColHomog& 
     const double fx0 = floor(x), fy0 = floor(y);
     const int x0 = (int)fx0, y0 = (int)fy0;
     uint32_t t[12];
-    if (s.ok) {
+    if (STAGED) {
         if (x0 < s.x_lo || y0 < s.y_lo || y0 + 1 >= s.y_lo + s.rows) return false;
         const uint32_t xb = (uint32_t)(x0 - s.x_lo) * 3u * (uint32_t)sizeof(T);
         if (xb + 6u * (uint32_t)sizeof(T) + 3u > 4u * (uint32_t)s.dw) return false;
@@ -452,6 +478,7 @@ __device__ __forceinline__ bool remap_fast(const FastFrame& ff, const ColHomog& 
             t[6 + c] = px[b00 + rs + c];
         }
     }
+    if (sizeof(T) == 1) return bilinear_taps_u8(t, x - fx0, y - fy0, band, o);
     return bilinear_taps(t, x - fx0, y - fy0, band, o);
 }
 
@@ -608,39 +635,48 @@ __global__ void __launch_bounds__(256, sizeof(T) == 1 ? 4 : 3) k_pass1(Pass1Args
         const T* fsrc = (const T*)f.src;
         const FrameDesc* fptr = a.frames + k;
         const ColHomog chm = fast ? col_homog(f.p, gx) : ColHomog{0.0, 0.0, 0.0};
-        // remap the region (remapRigidImage); gray (fillAverageFromLoaded) where alpha > 0
+        // remap the region (remapRigidImage); gray (fillAverageFromLoaded) where alpha > 0.  The
+        // loop is compiled once per tap source (staged box or global gather), box.ok being the
+        // same for the whole block.
+        auto remap_rows = [&](auto staged) {
 #pragma unroll 1
-        for (int r = wave; r < RH; r += 4) {
-            const int gy = Y0 - b + r;
-            uint32_t o[4] = {0, 0, 0, 0};
-            if (gx >= 0 && gy >= 0 && gx < W && gy < H) {
-                if (fast && remap_fast<T>(ff, chm, box, (const uint8_t*)stage, lead, fsrc, gy, band, o)) {
-                    o[3] = SampleMax<T>::i;
-                } else {
-                    const uint64_t e = frame_rgba_exact<T>(*fptr, gx, gy);
-                    o[0] = (uint32_t)e & 0xffffu;
-                    o[1] = (uint32_t)(e >> 16) & 0xffffu;
-                    o[2] = (uint32_t)(e >> 32) & 0xffffu;
-                    o[3] = (uint32_t)(e >> 48);
+            for (int r = wave; r < RH; r += 4) {
+                const int gy = Y0 - b + r;
+                uint32_t o[4] = {0, 0, 0, 0};
+                if (gx >= 0 && gy >= 0 && gx < W && gy < H) {
+                    if (fast && remap_fast<T, decltype(staged)::value>(ff, chm, box, (const uint8_t*)stage, lead, fsrc,
+                                                                       gy, band, o)) {
+                        o[3] = SampleMax<T>::i;
+                    } else {
+                        const uint64_t e = frame_rgba_exact<T>(*fptr, gx, gy);
+                        o[0] = (uint32_t)e & 0xffffu;
+                        o[1] = (uint32_t)(e >> 16) & 0xffffu;
+                        o[2] = (uint32_t)(e >> 32) & 0xffffu;
+                        o[3] = (uint32_t)(e >> 48);
+                    }
                 }
+                float g = -1.0f;
+                if (o[3] > 0) {
+                    const float sc = (float)((o[0] + o[1]) + o[2]);
+                    g = div_markstein(sc, SampleMax<T>::gray3, 1.0f / SampleMax<T>::gray3);
+                }
+                const PT c = P::make(o[0], o[1], o[2], o[3]);
+                if (keep_rgba && r >= b && r < RH - b && lane >= b && lane < P1C - b && gx < W && gy < H) {
+                    const size_t i = (size_t)gy * W + gx;  // an output pixel of the tile
+                    if (sizeof(T) == 1)
+                        ((uint32_t*)rgba)[i] = *(const uint32_t*)&c;
+                    else
+                        ((uint2*)rgba)[i] = *(const uint2*)&c;
+                }
+                gray[r * P1C + lane] = g;
+                alph[r * P1C + lane] = (uint16_t)o[3];
+                if (hyb) colour[r * P1C + lane] = c;
             }
-            float g = -1.0f;
-            if (o[3] > 0) {
-                const float sc = (float)((o[0] + o[1]) + o[2]);
-                g = div_markstein(sc, SampleMax<T>::gray3, 1.0f / SampleMax<T>::gray3);
-            }
-            const PT c = P::make(o[0], o[1], o[2], o[3]);
-            if (keep_rgba && r >= b && r < RH - b && lane >= b && lane < P1C - b && gx < W && gy < H) {
-                const size_t i = (size_t)gy * W + gx;  // an output pixel of the tile
-                if (sizeof(T) == 1)
-                    ((uint32_t*)rgba)[i] = *(const uint32_t*)&c;
-                else
-                    ((uint2*)rgba)[i] = *(const uint2*)&c;
-            }
-            gray[r * P1C + lane] = g;
-            alph[r * P1C + lane] = (uint16_t)o[3];
-            if (hyb) colour[r * P1C + lane] = c;
-        }
+        };
+        if (box.ok)
+            remap_rows(std::true_type{});
+        else
+            remap_rows(std::false_type{});
         corner(k + 1);
         lds_barrier();
         // the next frame's source box, in flight until the end of this frame
