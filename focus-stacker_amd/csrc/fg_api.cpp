@@ -211,6 +211,12 @@ struct fg_stack {
     uint8_t* out_dev;
     float* merge_gs;
     float* merge_gh;
+    // the cached stack's pass-2 + collapse launches, captured when the stack finishes a second time
+    // (reset, pooled) and replayed by every later finish: the launches and their arguments are
+    // fixed for the stack's life
+    hipGraphExec_t tail;
+    uint32_t tail_runs;   // cached finishes so far
+    bool tail_eager;      // capture failed once: stay eager
     // profiling: event pairs per phase, resolved on query
     bool profiling;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> ev[FG_PHASE_COUNT];
@@ -305,6 +311,7 @@ void free_stack(fg_stack* s) {
     }
     recycle_events(s);
     for (hipEvent_t e : s->ev_pool) (void)hipEventDestroy(e);
+    if (s->tail) (void)hipGraphExecDestroy(s->tail);
     for (void* p : s->allocs) {
         if (s->ext.alloc)
             s->ext.free(s->ext.user, p);
@@ -885,6 +892,9 @@ int fg_stack_pass2_frame_device(fg_stack* s, const fg_image_desc* d, const void*
     return pass2_frame(s, d, px, true, pose, roi);
 }
 
+static int finish_tail(fg_stack* s);
+static int tail_graph(fg_stack* s);
+
 static int finish(fg_stack* s) {
     if (!s) return fail(FG_ERR_INVALID_ARG, "null stack");
     if (s->finished) return FG_OK;
@@ -906,11 +916,25 @@ static int finish(fg_stack* s) {
         return FG_OK;
     }
     if (s->cached) {
+        // without profiling events or debug copies the schedule is kernel launches only; a stack
+        // finished once (the one-shot call) never pays for a capture
+        const bool reuse = s->tail_runs++ > 0;
+        if (reuse && !s->profiling && !s->accDbg && !s->tail_eager) {
+            rc = tail_graph(s);
+            if (rc) return rc;
+            s->finished = true;
+            return FG_OK;
+        }
         rc = run_pass2(s, 0, (int)s->opts.frame_count, 0);
         if (rc) return rc;
     } else if (s->pass2_done != s->opts.frame_count) {
         return fail(FG_ERR_STATE, "uncached stack: feed every frame through fg_stack_pass2_frame first");
     }
+    return finish_tail(s);
+}
+
+// collapse, quantize and the hybrid merge
+static int finish_tail(fg_stack* s) {
     PhaseMark pm(s, FG_PHASE_FINISH);
     if (s->accDbg)
         HIP_TRY(hipMemcpyAsync(s->accDbg, s->acc, s->lt.total * 3 * sizeof(float), hipMemcpyDeviceToDevice, s->stream),
@@ -927,6 +951,36 @@ static int finish(fg_stack* s) {
                 "hybrid merge");
     }
     s->finished = true;
+    return FG_OK;
+}
+
+// The cached stack's pass 2 and collapse as one HIP graph: captured from the stack's stream on its
+// second finish, instantiated, and launched by this and every later finish.  The captured launches
+// are kernels only (clears are kernels too), on buffers the stack owns for its whole life.  If the
+// runtime refuses the capture the stack stays on eager launches.
+static int tail_graph(fg_stack* s) {
+    if (!s->tail) {
+        hipGraph_t g = nullptr;
+        hipError_t e = hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal);
+        int rc = FG_OK;
+        if (e == hipSuccess) {
+            rc = run_pass2(s, 0, (int)s->opts.frame_count, 0);
+            if (rc == FG_OK) rc = finish_tail(s);
+            s->finished = false;
+            e = hipStreamEndCapture(s->stream, &g);
+        }
+        if (rc == FG_OK && e == hipSuccess && g) e = hipGraphInstantiate(&s->tail, g, nullptr, nullptr, 0);
+        if (g) (void)hipGraphDestroy(g);
+        if (rc != FG_OK || e != hipSuccess || !s->tail) {
+            (void)hipGetLastError();
+            if (s->tail) (void)hipGraphExecDestroy(s->tail);
+            s->tail = nullptr;
+            s->tail_eager = true;
+            rc = run_pass2(s, 0, (int)s->opts.frame_count, 0);
+            return rc ? rc : finish_tail(s);
+        }
+    }
+    HIP_TRY(hipGraphLaunch(s->tail, s->stream), "graph launch");
     return FG_OK;
 }
 

@@ -313,6 +313,38 @@ def test_levels_cap(ctx, fg, oracle):
     s.close()
 
 
+@pytest.mark.parametrize("method", ["pyramid", "hybrid"])
+def test_graph_replays_equal_eager(ctx, fg, method):
+    """A cached stack finished again runs pass 2 + collapse as a HIP graph captured on its second
+    finish; with profiling on it launches the same kernels eagerly.  Two frame sets alternate through one stack
+    (eager pass 1, reset and synchronisation between the runs), so a replay that kept stale data
+    or a captured clear that misbehaved on later replays would show."""
+    from oracle import make_pose
+    m = fg.METHOD_PYRAMID if method == "pyramid" else fg.METHOD_HYBRID
+    spec, fa, poses = _synthetic(dict(frames=4, width=160, height=120))
+    fb = [np.ascontiguousarray(f[:, ::-1]) for f in fa[::-1]]   # a different stack of the same geometry
+    roi = fg.common_overlap_roi([(spec.width, spec.height)] * 4, [fg.pose(p) for p in poses])
+    w, h = roi[2] - roi[0], roi[3] - roi[1]
+    s = fg.Stack(ctx, 4, w, h, method=m)
+
+    def run(frames):
+        s.reset()
+        for f, p in zip(frames, poses):
+            s.add_frame(f, fg.pose(p), roi)
+        out = s.finish()
+        s.sync()
+        return out
+
+    s.set_profiling(True)                       # eager launches
+    eager = {"a": run(fa), "b": run(fb)}
+    s.set_profiling(False)                      # captured on the next finish, replayed after
+    assert not np.array_equal(eager["a"], eager["b"])
+    for i, key in enumerate("ababab"):
+        out = run(fa if key == "a" else fb)
+        assert np.array_equal(out, eager[key]), f"run {i} ({key}) differs from the eager result"
+    s.close()
+
+
 @pytest.mark.parametrize("levels", [3, 4, 6, 7])
 def test_hybrid_levels_cap(ctx, fg, oracle, levels):
     """Hybrid with the pyramid capped: at 6 levels or fewer the hard range [2, 6) reaches the top
