@@ -313,6 +313,38 @@ def test_levels_cap(ctx, fg, oracle):
     s.close()
 
 
+@pytest.mark.parametrize("dtype", [np.uint8, np.uint16])
+def test_unaligned_device_frames(ctx, fg, dtype):
+    """Device frames that start off a dword boundary and end at the last byte of their
+    allocation: pass 1 stages the source through a dword-aligned view, assembling the dwords
+    that straddle either end of the frame from its own bytes.  The result equals the stack
+    fed from aligned buffers."""
+    import ctypes as C
+    kw = dict(frames=3, width=150, height=110) if dtype == np.uint8 else dict(frames=3, width=131, height=97,
+                                                                               dtype="u16")
+    spec, frames, poses = _synthetic(kw)
+    roi = fg.common_overlap_roi([(spec.width, spec.height)] * 3, [fg.pose(p) for p in poses])
+    _, ref = _run_stack(fg, ctx, frames, poses, roi)
+    st = fg.SAMPLE_U8 if dtype == np.uint8 else fg.SAMPLE_U16
+    sb = np.dtype(dtype).itemsize
+    desc = fg.ImageDesc(spec.width, spec.height, 3, 0, st)
+    w, h = roi[2] - roi[0], roi[3] - roi[1]
+    s = fg.Stack(ctx, 3, w, h, st)
+    allocs = []
+    for k, (f, p) in enumerate(zip(frames, poses)):
+        off = (k + 1) * sb                      # 1, 2, 3 bytes (u8) or 2, 4, 6 (u16, sample-aligned)
+        f = np.ascontiguousarray(f)
+        d = ctx.alloc(f.nbytes + off)           # the frame's last byte is the allocation's last byte
+        allocs.append(d)
+        fg.check(fg.lib().fg_copy_to_device(ctx.h, C.c_void_p(d + off), fg._p(f), f.nbytes))
+        s.add_frame_device(d + off, desc, fg.pose(p), roi)
+    out = s.finish()
+    s.close()
+    for d in allocs:
+        ctx.free(d)
+    assert np.array_equal(out, ref)
+
+
 @pytest.mark.parametrize("method", ["pyramid", "hybrid"])
 def test_graph_replays_equal_eager(ctx, fg, method):
     """A cached stack finished again runs pass 2 + collapse as a HIP graph captured on its second
