@@ -88,33 +88,36 @@ constexpr int P1C = 64;
 
 int pass1_rows(int window) { return window / 2 <= 4 ? 32 : 64; }
 
-// dynamic LDS of k_pass1 (byte offsets of every array in Pass1Lds): region gray and alpha (and
-// colour for the hybrid method), window-column sums, the tile's running norm sum and union
-// support and, for the hybrid method, its winning weight and colour
+// dynamic LDS of k_pass1 (byte offsets of every array in Pass1Lds): region gray and alpha, the
+// tile's running norm sum and union support and, for the hybrid method, its winning weight and
+// either the winner's frame slot (frames cached: the colour is read back from the frame cache at
+// the end) or the region colour and the winner's colour (frames not cached)
 struct Pass1Lds {
-    size_t gray, alpha, colour, stage, tS, tU, tB, tH, total;
+    size_t gray, alpha, colour, stage, tS, tU, tB, tH, tK, total;
 };
 // staged source box: 4 waves x StageGeom::NR rows of 64 x StageGeom::NC dwords
 constexpr size_t kStageBytesU8 = 4 * 12 * 64 * 1 * 4, kStageBytesU16 = 4 * 10 * 64 * 2 * 4;
-__host__ __device__ inline Pass1Lds pass1_lds(int rh, int th, int sb, bool hybrid) {
+__host__ __device__ inline Pass1Lds pass1_lds(int rh, int th, int sb, bool hybrid, bool cached) {
     Pass1Lds o;
     const size_t reg = (size_t)rh * P1C, til = (size_t)th * P1C, pt = sb == 1 ? 4 : 8;
+    const bool colour = hybrid && !cached;
     size_t at = 0;
     o.stage = at;  at += sb == 1 ? kStageBytesU8 : kStageBytesU16;
     o.gray = at;   at += reg * 4;
     o.tS = at;     at += til * 4;
     o.tB = at;     at += hybrid ? til * 4 : 0;
-    o.colour = at; at += hybrid ? reg * pt : 0;
-    o.tH = at;     at += hybrid ? til * pt : 0;
+    o.colour = at; at += colour ? reg * pt : 0;
+    o.tH = at;     at += colour ? til * pt : 0;
+    o.tK = at;     at += hybrid && cached ? til * 2 : 0;
     o.alpha = at;  at += reg * 2;
     o.tU = at;     at += til;
     o.total = (at + 15) & ~(size_t)15;
     return o;
 }
 
-size_t pass1_lds_bytes(int window, int sb, bool hybrid) {
+size_t pass1_lds_bytes(int window, int sb, bool hybrid, bool cached) {
     const int b = window / 2, rh = pass1_rows(window);
-    return pass1_lds(rh, rh - 2 * b, sb, hybrid).total;
+    return pass1_lds(rh, rh - 2 * b, sb, hybrid, cached).total;
 }
 
 // 1 / d to within a few ulp: hardware estimate and two Newton steps.
@@ -540,7 +543,10 @@ __global__ void __launch_bounds__(256, sizeof(T) == 1 ? 4 : 3) k_pass1(Pass1Args
     const int rpw = (th + 3) / 4;
     const bool acc = a.norm_sum != nullptr;
     const bool hyb = a.best != nullptr;
-    const Pass1Lds L = pass1_lds(RH, th, (int)sizeof(T), hyb);
+    // hybrid with the frames cached: the winner is kept as its frame slot, its colour read back
+    // from the frame cache when the tile is written (the RGBA the region colour would have held)
+    const bool hyb_slot = hyb && a.rgba != nullptr;
+    const Pass1Lds L = pass1_lds(RH, th, (int)sizeof(T), hyb, a.rgba != nullptr);
     float* gray = (float*)(smem + L.gray);            // [RH][64] gray, -1 where alpha == 0 or outside
     uint16_t* alph = (uint16_t*)(smem + L.alpha);     // [RH][64] alpha
     PT* colour = (PT*)(smem + L.colour);              // [RH][64] hybrid: packed rgb
@@ -548,7 +554,8 @@ __global__ void __launch_bounds__(256, sizeof(T) == 1 ? 4 : 3) k_pass1(Pass1Args
     float* tS = (float*)(smem + L.tS);                // [th][64] norm sum of the tile
     uint8_t* tU = smem + L.tU;                        // [th][64] union support
     float* tB = (float*)(smem + L.tB);                // [th][64] hybrid: best weight
-    PT* tH = (PT*)(smem + L.tH);                      // [th][64] hybrid: its colour
+    PT* tH = (PT*)(smem + L.tH);                      // [th][64] hybrid, frames not cached: its colour
+    uint16_t* tK = (uint16_t*)(smem + L.tK);          // [th][64] hybrid, frames cached: its slot
     __shared__ double corners[8];                     // next frame's region corners in its source
 
     const int W = (int)a.w, H = (int)a.h;
@@ -599,18 +606,19 @@ __global__ void __launch_bounds__(256, sizeof(T) == 1 ? 4 : 3) k_pass1(Pass1Args
                 sv = a.norm_sum[i];
                 uv = a.union_sup[i];
             }
-            if (hyb) {
+            if (hyb) bv = a.best[i];
+            if (hyb && !hyb_slot) {
                 const T* hd = (const T*)a.hard + i * 3;
-                bv = a.best[i];
                 hv = P::make(hd[0], hd[1], hd[2], 0);
             }
         }
         tS[r * P1C + lane] = sv;
         tU[r * P1C + lane] = (uint8_t)uv;
-        if (hyb) {
-            tB[r * P1C + lane] = bv;
+        if (hyb) tB[r * P1C + lane] = bv;
+        if (hyb_slot)
+            tK[r * P1C + lane] = 0xffffu;                 // no winner in this launch yet
+        else if (hyb)
             tH[r * P1C + lane] = hv;
-        }
     }
 
     // stage the first frame
@@ -670,7 +678,7 @@ __global__ void __launch_bounds__(256, sizeof(T) == 1 ? 4 : 3) k_pass1(Pass1Args
                 }
                 gray[r * P1C + lane] = g;
                 alph[r * P1C + lane] = (uint16_t)o[3];
-                if (hyb) colour[r * P1C + lane] = c;
+                if (hyb && !hyb_slot) colour[r * P1C + lane] = c;
             }
         };
         if (box.ok)
@@ -753,7 +761,10 @@ __global__ void __launch_bounds__(256, sizeof(T) == 1 ? 4 : 3) k_pass1(Pass1Args
             }
             if (hyb && wv > tB[t]) {  // updateRange: strictly greater wins, first frame on ties
                 tB[t] = wv;
-                tH[t] = colour[(r + b) * P1C + lane];
+                if (hyb_slot)
+                    tK[t] = (uint16_t)slot;
+                else
+                    tH[t] = colour[(r + b) * P1C + lane];
             }
         }
         // the next frame's box: its loads were issued before the contrast, the remap of this
@@ -762,6 +773,9 @@ __global__ void __launch_bounds__(256, sizeof(T) == 1 ? 4 : 3) k_pass1(Pass1Args
         if (box.ok) stage_store<T>(box, lane, wave, pf, stage);
         lds_barrier();
     }
+    // the winners' RGBA were stored to the frame cache by whichever wave remapped their row: a
+    // workgroup barrier with a global-memory fence makes them visible to the reading thread
+    if (hyb_slot) __syncthreads();
 
 #pragma unroll 1
     for (int r = r0; r < r1; r++) {
@@ -773,10 +787,19 @@ __global__ void __launch_bounds__(256, sizeof(T) == 1 ? 4 : 3) k_pass1(Pass1Args
             a.norm_sum[i] = tS[t];
             a.union_sup[i] = (uint8_t)tU[t];
         }
-        if (hyb) {
+        if (hyb) a.best[i] = tB[t];
+        if (hyb_slot) {
+            const uint32_t k = tK[t];
+            if (k != 0xffffu) {  // the winner's RGBA, stored by this block's remap of that frame
+                const T* px = (const T*)a.rgba + ((size_t)k * fpx + i) * 4;
+                T* hd = (T*)a.hard + i * 3;
+                hd[0] = px[0];
+                hd[1] = px[1];
+                hd[2] = px[2];
+            }
+        } else if (hyb) {
             T* hd = (T*)a.hard + i * 3;
             const PT h = tH[t];
-            a.best[i] = tB[t];
             hd[0] = (T)P::ch(h, 0);
             hd[1] = (T)P::ch(h, 1);
             hd[2] = (T)P::ch(h, 2);
@@ -787,7 +810,7 @@ __global__ void __launch_bounds__(256, sizeof(T) == 1 ? 4 : 3) k_pass1(Pass1Args
 template <typename T, int RH>
 static hipError_t launch_pass1_t(const Pass1Args& a, hipStream_t s) {
     const int b = (int)a.window / 2, th = RH - 2 * b, tw = P1C - 2 * b;
-    const size_t lds = pass1_lds_bytes((int)a.window, (int)sizeof(T), a.best != nullptr);
+    const size_t lds = pass1_lds_bytes((int)a.window, (int)sizeof(T), a.best != nullptr, a.rgba != nullptr);
     if (lds > 65536) {
         hipError_t e =
             hipFuncSetAttribute((const void*)k_pass1<T, RH>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);

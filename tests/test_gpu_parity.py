@@ -391,16 +391,22 @@ def test_hybrid_levels_cap(ctx, fg, oracle, levels):
     s.close()
 
 
-def test_uncached_second_pass_equals_cached(ctx, fg, oracle):
-    """With a memory budget too small for the frame cache the caller feeds the frames again."""
+@pytest.mark.parametrize("method", ["pyramid", "hybrid"])
+def test_uncached_second_pass_equals_cached(ctx, fg, oracle, method):
+    """With a memory budget too small for the frame cache the caller feeds the frames again.  For
+    the hybrid, the cached stack keeps each winner as a frame slot and reads its colour back from
+    the frame cache; the uncached one keeps the colour itself."""
+    m = fg.METHOD_PYRAMID if method == "pyramid" else fg.METHOD_HYBRID
     spec, frames, poses = _synthetic(dict(frames=3, width=100, height=80))
     from oracle import make_pose
     opo = [make_pose(**p) for p in poses]
     roi = oracle.common_overlap_roi([(spec.width, spec.height)] * spec.frames, opo)
-    s1, out1 = _run_stack(fg, ctx, frames, poses, roi)
-    s2, out2 = _run_stack(fg, ctx, frames, poses, roi, memory_fraction=1e-9)
+    s1, out1 = _run_stack(fg, ctx, frames, poses, roi, method=m)
+    s2, out2 = _run_stack(fg, ctx, frames, poses, roi, memory_fraction=1e-9, method=m)
     assert not s1.needs_pass2_frames() and s2.needs_pass2_frames()
     assert np.array_equal(out1, out2)
+    if method == "hybrid":
+        assert np.array_equal(s1.readback("hard"), s2.readback("hard"))
     s1.close()
     s2.close()
 
