@@ -226,15 +226,19 @@ def test_hybrid_matches_oracle(ctx, fg, oracle, sharpness):
     s.close()
 
 
-@pytest.mark.parametrize("window", [3, 7])
-def test_contrast_window_sizes(ctx, fg, oracle, window):
-    spec, frames, poses = _synthetic(dict(frames=3, width=120, height=90))
+@pytest.mark.parametrize("window,dtype", [(3, "u8"), (7, "u8"), (9, "u8"), (11, "u8"), (21, "u8"), (31, "u8"),
+                                          (11, "u16"), (31, "u16")])
+def test_contrast_window_sizes(ctx, fg, oracle, window, dtype):
+    """--contrast-window-size 3..31 (src/fuse/config.zig:295-299): windows up to 7 use the fixed
+    cross-lane sums, wider ones the generic sum, and from 11 on pass 1 runs 64-row tiles."""
+    spec, frames, poses = _synthetic(dict(frames=3, width=120, height=90, dtype=dtype))
     from oracle import make_pose
     opo = [make_pose(**p) for p in poses]
     roi = oracle.common_overlap_roi([(spec.width, spec.height)] * spec.frames, opo)
-    s, out = _run_stack(fg, ctx, frames, poses, roi, window=window)
-    _img_close(out, oracle.stack(frames, opo, roi, window=window))
-    s.close()
+    for method, om in ((fg.METHOD_PYRAMID, 0), (fg.METHOD_HYBRID, 1)):
+        s, out = _run_stack(fg, ctx, frames, poses, roi, window=window, method=method)
+        _img_close(out, oracle.stack(frames, opo, roi, window=window, method=om))
+        s.close()
 
 
 @pytest.mark.parametrize("alpha", [False, True])
