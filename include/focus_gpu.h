@@ -172,7 +172,9 @@ int fg_stack_pass2_frame_device(fg_stack* stack, const fg_image_desc* desc, cons
 
 /* Pass 2 (if cached), collapse, hybrid merge; writes W*H*3 samples (extra_channels = 0,
  * src/fuse/pyramid.zig:252-253).  The _device form leaves the result in device memory and
- * does not wait for it. */
+ * does not wait for it.  From a cached stack's second finish on (after fg_stack_reset), pass 2
+ * and the collapse run as a HIP graph captured from the stack's stream on that finish; with
+ * profiling on, or debug buffers kept, they are launched one by one. */
 int fg_stack_finish(fg_stack* stack, void* out_px);
 int fg_stack_finish_device(fg_stack* stack, void* d_out);
 
