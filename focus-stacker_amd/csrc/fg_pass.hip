@@ -115,6 +115,12 @@ __host__ __device__ inline Pass1Lds pass1_lds(int rh, int th, int sb, bool hybri
     return o;
 }
 
+// Hybrid winners are kept as 16-bit frame slots (0xffff = none) when the frames of the launch are
+// cached; the kernel and its launch decide this the same way.
+__host__ __device__ inline bool pass1_slot_winners(const Pass1Args& a) {
+    return a.rgba != nullptr && a.k1 - a.slot_base < 0xffff;
+}
+
 size_t pass1_lds_bytes(int window, int sb, bool hybrid, bool cached) {
     const int b = window / 2, rh = pass1_rows(window);
     return pass1_lds(rh, rh - 2 * b, sb, hybrid, cached).total;
@@ -545,8 +551,8 @@ __global__ void __launch_bounds__(256, sizeof(T) == 1 ? 4 : 3) k_pass1(Pass1Args
     const bool hyb = a.best != nullptr;
     // hybrid with the frames cached: the winner is kept as its frame slot, its colour read back
     // from the frame cache when the tile is written (the RGBA the region colour would have held)
-    const bool hyb_slot = hyb && a.rgba != nullptr;
-    const Pass1Lds L = pass1_lds(RH, th, (int)sizeof(T), hyb, a.rgba != nullptr);
+    const bool hyb_slot = hyb && pass1_slot_winners(a);
+    const Pass1Lds L = pass1_lds(RH, th, (int)sizeof(T), hyb, pass1_slot_winners(a));
     float* gray = (float*)(smem + L.gray);            // [RH][64] gray, -1 where alpha == 0 or outside
     uint16_t* alph = (uint16_t*)(smem + L.alpha);     // [RH][64] alpha
     PT* colour = (PT*)(smem + L.colour);              // [RH][64] hybrid: packed rgb
@@ -810,7 +816,7 @@ __global__ void __launch_bounds__(256, sizeof(T) == 1 ? 4 : 3) k_pass1(Pass1Args
 template <typename T, int RH>
 static hipError_t launch_pass1_t(const Pass1Args& a, hipStream_t s) {
     const int b = (int)a.window / 2, th = RH - 2 * b, tw = P1C - 2 * b;
-    const size_t lds = pass1_lds_bytes((int)a.window, (int)sizeof(T), a.best != nullptr, a.rgba != nullptr);
+    const size_t lds = pass1_lds_bytes((int)a.window, (int)sizeof(T), a.best != nullptr, pass1_slot_winners(a));
     if (lds > 65536) {
         hipError_t e =
             hipFuncSetAttribute((const void*)k_pass1<T, RH>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
