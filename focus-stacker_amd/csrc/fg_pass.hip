@@ -157,57 +157,6 @@ __device__ __forceinline__ void lds_barrier() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
-// One pixel of the homography fast path between its gather and its arithmetic: the bilinear
-// weights and the 12 source samples (rows y0, y0+1 x columns x0, x0+1 x rgb).
-struct FastTaps {
-    double w00, w10, w01, w11;
-    uint32_t t[12];
-    bool ok;  // interior of the source: the taps are valid
-};
-
-// Gather of the interior case of samplePixelBilinear (src/remap.zig:788-839, 886-937) at
-// (x, y), 0 <= x < w-1 and 0 <= y < h-1, straight from the source in global memory.
-template <typename T>
-__device__ __forceinline__ void gather_at(const T* __restrict__ src, uint32_t sw, double x, double y, FastTaps& g) {
-    const __attribute__((address_space(1))) T* px = as_global(src);
-    const double fx0 = floor(x), fy0 = floor(y);
-    const uint32_t x0 = (uint32_t)(int)fx0, y0 = (uint32_t)(int)fy0;
-    const double fx = x - fx0, fy = y - fy0;
-    const double omx = 1.0 - fx, omy = 1.0 - fy;
-    g.w00 = omx * omy;
-    g.w10 = fx * omy;
-    g.w01 = omx * fy;
-    g.w11 = fx * fy;
-    const uint32_t rs = sw * 3u;
-    const uint32_t b00 = y0 * rs + x0 * 3u;
-#pragma unroll
-    for (int c = 0; c < 6; c++) {
-        g.t[c] = px[b00 + c];
-        g.t[6 + c] = px[b00 + rs + c];
-    }
-    g.ok = true;
-}
-
-// The arithmetic of the interior case on gathered taps: the reference's f64 weighted sums, the
-// quotient by the weight sum through a reciprocal.  Returns false if a channel lies within
-// `band` of a rounding tie (the caller then redoes the pixel exactly).
-__device__ __forceinline__ bool resolve_fast(const FastTaps& g, double band, uint32_t o[3]) {
-    const double sup = ((g.w00 + g.w10) + g.w01) + g.w11;
-    const double inv = rcp_f64(sup);
-    bool ok = true;
-#pragma unroll
-    for (int c = 0; c < 3; c++) {
-        const double v = (((double)g.t[c] * g.w00 + (double)g.t[3 + c] * g.w10) + (double)g.t[6 + c] * g.w01) +
-                         (double)g.t[9 + c] * g.w11;
-        const double t = v * inv + 0.5;
-        const double fl = floor(t);
-        const double d = t - fl;
-        ok = ok && d > band && d < 1.0 - band;
-        o[c] = (uint32_t)fl;
-    }
-    return ok;
-}
-
 // ---- source staging: the footprint of a block's region in one frame's source ----
 // Homography frames have a convex footprint (the image of the region's corners), so its
 // bounding box, read row by row with coalesced dword loads, holds every interior tap.  The box
@@ -341,41 +290,6 @@ __device__ __forceinline__ void stage_store(const StageBox& s, int lane, int wav
             if (row < s.rows && col < s.dw) stage[row * s.dw + col] = pf[ir * G::NC + ic];
         }
     }
-}
-
-// gather_fast with the taps read from the staged box; ok = false when the pixel is not in the
-// source interior or, through rounding, a tap falls outside the box.
-template <typename T>
-__device__ __forceinline__ void gather_staged(const FrameDesc& f, const StageBox& s, const uint8_t* stage,
-                                              uint32_t lead, double x, double y, double wlim, double hlim,
-                                              FastTaps& g) {
-    g.ok = false;
-    if (!(x >= 0.0 && y >= 0.0 && x < wlim && y < hlim)) return;
-    const double fx0 = floor(x), fy0 = floor(y);
-    const int x0 = (int)fx0, y0 = (int)fy0;
-    if (x0 < s.x_lo || y0 < s.y_lo || y0 + 1 >= s.y_lo + s.rows) return;
-    const uint32_t xb = (uint32_t)(x0 - s.x_lo) * 3u * (uint32_t)sizeof(T);
-    if (xb + 6u * (uint32_t)sizeof(T) + 3u > 4u * (uint32_t)s.dw) return;
-    const uint32_t r0 = (uint32_t)(y0 - s.y_lo);
-    const uint32_t a0 = r0 * 4u * (uint32_t)s.dw + (stage_row_byte<T>(f, s, y0, lead) & 3u) + xb;
-    const uint32_t a1 = (r0 + 1u) * 4u * (uint32_t)s.dw + (stage_row_byte<T>(f, s, y0 + 1, lead) & 3u) + xb;
-    const double fx = x - fx0, fy = y - fy0;
-    const double omx = 1.0 - fx, omy = 1.0 - fy;
-    g.w00 = omx * omy;
-    g.w10 = fx * omy;
-    g.w01 = omx * fy;
-    g.w11 = fx * fy;
-#pragma unroll
-    for (int c = 0; c < 6; c++) {
-        if (sizeof(T) == 1) {
-            g.t[c] = stage[a0 + c];
-            g.t[6 + c] = stage[a1 + c];
-        } else {
-            g.t[c] = *(const uint16_t*)&stage[a0 + 2 * c];
-            g.t[6 + c] = *(const uint16_t*)&stage[a1 + 2 * c];
-        }
-    }
-    g.ok = true;
 }
 
 // A frame's fast-path constants, copied out of its descriptor once per frame (the descriptor sits
