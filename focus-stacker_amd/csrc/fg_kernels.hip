@@ -52,30 +52,6 @@ hipError_t launch_remap(const void* src, int sb, int ch, uint32_t sw, uint32_t s
     return hipGetLastError();
 }
 
-template <typename T>
-__global__ void k_to_rgba(const T* __restrict__ src, int extra, size_t n, T* __restrict__ dst) {
-    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    int st = 3 + extra;
-    T r = src[i * st], g = src[i * st + 1], b = src[i * st + 2];
-    T a = extra ? src[i * st + 3] : (T)SampleMax<T>::i;
-    dst[i * 4] = r;
-    dst[i * 4 + 1] = g;
-    dst[i * 4 + 2] = b;
-    dst[i * 4 + 3] = a;
-}
-
-hipError_t launch_to_rgba(const void* src, int sb, int extra, uint32_t w, uint32_t h, void* dst, hipStream_t s) {
-    size_t n = (size_t)w * h;
-    dim3 grid((unsigned)((n + 255) / 256));
-    if (sb == 1)
-        hipLaunchKernelGGL((k_to_rgba<uint8_t>), grid, dim3(256), 0, s, (const uint8_t*)src, extra, n, (uint8_t*)dst);
-    else
-        hipLaunchKernelGGL((k_to_rgba<uint16_t>), grid, dim3(256), 0, s, (const uint16_t*)src, extra, n,
-                           (uint16_t*)dst);
-    return hipGetLastError();
-}
-
 // ---------------------------------------------------------------------------
 // collapse (collapseToImageWithJobsAndDebug src/fuse/pyramid.zig:223-269)
 // ---------------------------------------------------------------------------
