@@ -173,6 +173,8 @@ CASES = {
     "odd_u8": dict(frames=3, width=99, height=67),
     "clamp_regime_u8": dict(frames=3, width=720, height=560),   # level 0 and 1 expand in the >512 regime
     "u16": dict(frames=3, width=131, height=97, dtype="u16"),
+    "clamp_regime_u16": dict(frames=3, width=720, height=560, dtype="u16"),
+    "host_ring_u8": dict(frames=11, width=160, height=120),       # more host frames than upload slots
     "tall": dict(frames=3, width=40, height=150),                # source levels one pixel wide
 }
 
@@ -315,6 +317,24 @@ def test_levels_cap(ctx, fg, oracle):
     assert len(s.levels()) == 4
     _img_close(out, oracle.stack(frames, opo, roi, max_levels=4))
     s.close()
+
+
+@pytest.mark.parametrize("dtype", [np.uint8, np.uint16])
+def test_translation_poses_stack(ctx, fg, oracle, dtype):
+    """Poses with translation terms (src/optimize.zig:4009-4043): every pixel of pass 1 takes the
+    exact remap path instead of the homography fast path."""
+    from oracle import make_pose
+    kw = dict(frames=3, width=140, height=100) if dtype == np.uint8 else dict(frames=3, width=140, height=100,
+                                                                               dtype="u16")
+    spec, frames, poses = _synthetic(kw)
+    tr = [dict(), dict(trans_x=0.02, trans_y=-0.015, trans_z=0.01, translation_plane_yaw=0.01),
+          dict(trans_x=-0.01, trans_y=0.02, trans_z=-0.012, translation_plane_pitch=-0.01)]
+    poses = [dict(p, **t) for p, t in zip(poses, tr)]
+    opo = [make_pose(**p) for p in poses]
+    roi = oracle.common_overlap_roi([(spec.width, spec.height)] * spec.frames, opo)
+    s, out = _run_stack(fg, ctx, frames, poses, roi)
+    s.close()
+    _img_close(out, oracle.stack(frames, opo, roi))
 
 
 @pytest.mark.parametrize("dtype", [np.uint8, np.uint16])
