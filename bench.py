@@ -10,7 +10,7 @@ Frames are generated on the GPU and stay resident in HBM; the timed region is K 
 Output: one JSON line.  roofline: the longer of the two dominant launches (k_pass1, the level-0
 k_pass2), its per frame-pixel algorithmic bytes (kernel_models) times the frame-pixels of one
 launch over its mean duration from HIP events on the stack's stream in the timed region.
-cpu_baseline: the single-threaded C oracle on a bounded slice of the same stack.
+cpu_baseline: the C oracle on 16 host threads over a bounded sample of the same stack.
 """
 import argparse
 import json

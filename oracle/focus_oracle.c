@@ -1,7 +1,7 @@
 /*
  * focus_oracle.c — CPU restatement of the reference's remap + pyramid-contrast
  * path.  TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke()
- * and bench.py's cpu_baseline leg as the checker; the product library never
+ * and bench.py's cpu_baseline leg, to check results; the product library never
  * links or calls it.
  *
  * Each function names the reference lines (asingleoat/focus-stacker, paths

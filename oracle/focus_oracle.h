@@ -1,6 +1,6 @@
 /*
  * focus_oracle.h — CPU restatement of the focus-stacker remap + pyramid-contrast
- * path, used ONLY as the test checker (tests/, __graft_entry__.smoke(), and the
+ * path, used ONLY to check results (tests/, __graft_entry__.smoke(), and the
  * cpu_baseline leg of bench.py).  Nothing in the product library links it.
  *
  * Every function follows a reference file:line (cited in focus_oracle.c).  The

@@ -1,7 +1,7 @@
 """ctypes front end of the CPU oracle (oracle/focus_oracle.c).
 
 TEST INFRASTRUCTURE ONLY: tests/, __graft_entry__.smoke() and bench.py's
-cpu_baseline leg use it as the checker.  It restates the reference algorithm;
+cpu_baseline leg use it, to check results.  It restates the reference algorithm;
 it is never the thing measured or shipped.
 
 `Oracle(fp64=False)` wraps liboracle.so (the reference's f32/f64 mix);

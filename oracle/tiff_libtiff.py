@@ -1,6 +1,6 @@
 """The reference's TIFF writer replayed through the system libtiff (ctypes).
 
-TEST INFRASTRUCTURE ONLY: tests/ use it as the checker of fg_write_tiff; it is never
+TEST INFRASTRUCTURE ONLY: tests/ use it to check fg_write_tiff; it is never
 the thing measured or shipped.
 
 write_tiff() makes the libtiff calls of openTiffWriter (src/image_io.zig:100-143) in

@@ -1,4 +1,4 @@
-"""Pins the CPU oracle before it is trusted as the checker of the HIP path.
+"""Pins the CPU oracle before it is trusted to check the HIP path.
 
 Three kinds of checks, each chosen so that a plausible slip in the oracle
 (dropped term, wrong index, transposed operand, wrong divisor) fails one:
