@@ -89,28 +89,28 @@ constexpr int P1C = 64;
 int pass1_rows(int window) { return window / 2 <= 4 ? 32 : 64; }
 
 // dynamic LDS of k_pass1 (byte offsets of every array in Pass1Lds): region gray and alpha, the
-// tile's running norm sum and union support and, for the hybrid method, its winning weight and
-// either the winner's frame slot (frames cached: the colour is read back from the frame cache at
-// the end) or the region colour and the winner's colour (frames not cached)
+// tile's running norm sum and union support (not for the single-pass methods) and, for the hybrid
+// and hardmask methods, the winning weight and either the winner's frame slot (frames cached: its
+// colour is read back from the frame cache at the end) or the region colour (frames not cached:
+// a new winner's colour is written to the hard image right away)
 struct Pass1Lds {
-    size_t gray, alpha, colour, stage, tS, tU, tB, tH, tK, total;
+    size_t gray, alpha, colour, stage, tS, tU, tB, tK, total;
 };
 // staged source box: 4 waves x StageGeom::NR rows of 64 x StageGeom::NC dwords
 constexpr size_t kStageBytesU8 = 4 * 12 * 64 * 1 * 4, kStageBytesU16 = 4 * 10 * 64 * 2 * 4;
-__host__ __device__ inline Pass1Lds pass1_lds(int rh, int th, int sb, bool hybrid, bool cached) {
+__host__ __device__ inline Pass1Lds pass1_lds(int rh, int th, int sb, bool hybrid, bool cached, bool acc) {
     Pass1Lds o;
     const size_t reg = (size_t)rh * P1C, til = (size_t)th * P1C, pt = sb == 1 ? 4 : 8;
     const bool colour = hybrid && !cached;
     size_t at = 0;
     o.stage = at;  at += sb == 1 ? kStageBytesU8 : kStageBytesU16;
     o.gray = at;   at += reg * 4;
-    o.tS = at;     at += til * 4;
+    o.tS = at;     at += acc ? til * 4 : 0;
     o.tB = at;     at += hybrid ? til * 4 : 0;
     o.colour = at; at += colour ? reg * pt : 0;
-    o.tH = at;     at += colour ? til * pt : 0;
     o.tK = at;     at += hybrid && cached ? til * 2 : 0;
     o.alpha = at;  at += reg * 2;
-    o.tU = at;     at += til;
+    o.tU = at;     at += acc ? til : 0;
     o.total = (at + 15) & ~(size_t)15;
     return o;
 }
@@ -121,9 +121,9 @@ __host__ __device__ inline bool pass1_slot_winners(const Pass1Args& a) {
     return a.rgba != nullptr && a.k1 - a.slot_base < 0xffff;
 }
 
-size_t pass1_lds_bytes(int window, int sb, bool hybrid, bool cached) {
+size_t pass1_lds_bytes(int window, int sb, bool hybrid, bool cached, bool acc) {
     const int b = window / 2, rh = pass1_rows(window);
-    return pass1_lds(rh, rh - 2 * b, sb, hybrid, cached).total;
+    return pass1_lds(rh, rh - 2 * b, sb, hybrid, cached, acc).total;
 }
 
 // 1 / d to within a few ulp: hardware estimate and two Newton steps.
@@ -452,7 +452,9 @@ __device__ __forceinline__ V lane_window_sum(V x, int b) {
     return s;
 }
 
-template <typename T, int RH>
+// COL: hybrid or hardmask winners without a frame cache (the region colour in LDS, new winners
+// written to the hard image at once); compiled apart so that the other launches carry none of it.
+template <typename T, int RH, bool COL>
 __global__ void __launch_bounds__(256, sizeof(T) == 1 ? 4 : 3) k_pass1(Pass1Args a) {
     using P = Pack<T>;
     using PT = typename P::type;
@@ -465,8 +467,8 @@ __global__ void __launch_bounds__(256, sizeof(T) == 1 ? 4 : 3) k_pass1(Pass1Args
     const bool hyb = a.best != nullptr;
     // hybrid with the frames cached: the winner is kept as its frame slot, its colour read back
     // from the frame cache when the tile is written (the RGBA the region colour would have held)
-    const bool hyb_slot = hyb && pass1_slot_winners(a);
-    const Pass1Lds L = pass1_lds(RH, th, (int)sizeof(T), hyb, pass1_slot_winners(a));
+    const bool hyb_slot = !COL && hyb;  // the launch picks COL whenever there are winners and no cache
+    const Pass1Lds L = pass1_lds(RH, th, (int)sizeof(T), hyb, pass1_slot_winners(a), a.norm_sum != nullptr);
     float* gray = (float*)(smem + L.gray);            // [RH][64] gray, -1 where alpha == 0 or outside
     uint16_t* alph = (uint16_t*)(smem + L.alpha);     // [RH][64] alpha
     PT* colour = (PT*)(smem + L.colour);              // [RH][64] hybrid: packed rgb
@@ -474,7 +476,6 @@ __global__ void __launch_bounds__(256, sizeof(T) == 1 ? 4 : 3) k_pass1(Pass1Args
     float* tS = (float*)(smem + L.tS);                // [th][64] norm sum of the tile
     uint8_t* tU = smem + L.tU;                        // [th][64] union support
     float* tB = (float*)(smem + L.tB);                // [th][64] hybrid: best weight
-    PT* tH = (PT*)(smem + L.tH);                      // [th][64] hybrid, frames not cached: its colour
     uint16_t* tK = (uint16_t*)(smem + L.tK);          // [th][64] hybrid, frames cached: its slot
     __shared__ double corners[8];                     // next frame's region corners in its source
 
@@ -519,7 +520,6 @@ __global__ void __launch_bounds__(256, sizeof(T) == 1 ? 4 : 3) k_pass1(Pass1Args
         const int y = Y0 + r;
         float sv = 0.0f, bv = 0.0f;
         uint32_t uv = 0;
-        PT hv = P::make(0, 0, 0, 0);
         if (col_ok && y < H) {
             const size_t i = (size_t)y * W + ox;
             if (acc) {
@@ -527,18 +527,13 @@ __global__ void __launch_bounds__(256, sizeof(T) == 1 ? 4 : 3) k_pass1(Pass1Args
                 uv = a.union_sup[i];
             }
             if (hyb) bv = a.best[i];
-            if (hyb && !hyb_slot) {
-                const T* hd = (const T*)a.hard + i * 3;
-                hv = P::make(hd[0], hd[1], hd[2], 0);
-            }
         }
-        tS[r * P1C + lane] = sv;
-        tU[r * P1C + lane] = (uint8_t)uv;
+        if (acc) {  // the single-pass methods have no norm sum: tS and tU take no LDS
+            tS[r * P1C + lane] = sv;
+            tU[r * P1C + lane] = (uint8_t)uv;
+        }
         if (hyb) tB[r * P1C + lane] = bv;
-        if (hyb_slot)
-            tK[r * P1C + lane] = 0xffffu;                 // no winner in this launch yet
-        else if (hyb)
-            tH[r * P1C + lane] = hv;
+        if (hyb_slot) tK[r * P1C + lane] = 0xffffu;   // no winner in this launch yet
     }
 
     // stage the first frame
@@ -598,7 +593,7 @@ __global__ void __launch_bounds__(256, sizeof(T) == 1 ? 4 : 3) k_pass1(Pass1Args
                 }
                 gray[r * P1C + lane] = g;
                 alph[r * P1C + lane] = (uint16_t)o[3];
-                if (hyb && !hyb_slot) colour[r * P1C + lane] = c;
+                if (COL) colour[r * P1C + lane] = c;
             }
         };
         if (box.ok)
@@ -681,10 +676,15 @@ __global__ void __launch_bounds__(256, sizeof(T) == 1 ? 4 : 3) k_pass1(Pass1Args
             }
             if (hyb && wv > tB[t]) {  // updateRange: strictly greater wins, first frame on ties
                 tB[t] = wv;
-                if (hyb_slot)
+                if (!COL) {
                     tK[t] = (uint16_t)slot;
-                else
-                    tH[t] = colour[(r + b) * P1C + lane];
+                } else {  // only this thread writes this pixel of the hard image
+                    const PT c = colour[(r + b) * P1C + lane];
+                    T* hd = (T*)a.hard + i * 3;
+                    hd[0] = (T)P::ch(c, 0);
+                    hd[1] = (T)P::ch(c, 1);
+                    hd[2] = (T)P::ch(c, 2);
+                }
             }
         }
         // the next frame's box: its loads were issued before the contrast, the remap of this
@@ -717,28 +717,29 @@ __global__ void __launch_bounds__(256, sizeof(T) == 1 ? 4 : 3) k_pass1(Pass1Args
                 hd[1] = px[1];
                 hd[2] = px[2];
             }
-        } else if (hyb) {
-            T* hd = (T*)a.hard + i * 3;
-            const PT h = tH[t];
-            hd[0] = (T)P::ch(h, 0);
-            hd[1] = (T)P::ch(h, 1);
-            hd[2] = (T)P::ch(h, 2);
         }
     }
 }
 
-template <typename T, int RH>
-static hipError_t launch_pass1_t(const Pass1Args& a, hipStream_t s) {
+template <typename T, int RH, bool COL>
+static hipError_t launch_pass1_k(const Pass1Args& a, size_t lds, hipStream_t s) {
     const int b = (int)a.window / 2, th = RH - 2 * b, tw = P1C - 2 * b;
-    const size_t lds = pass1_lds_bytes((int)a.window, (int)sizeof(T), a.best != nullptr, pass1_slot_winners(a));
     if (lds > 65536) {
         hipError_t e =
-            hipFuncSetAttribute((const void*)k_pass1<T, RH>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+            hipFuncSetAttribute((const void*)k_pass1<T, RH, COL>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         if (e != hipSuccess) return e;
     }
     dim3 grid((a.w + tw - 1) / tw, (a.h + th - 1) / th);
-    hipLaunchKernelGGL((k_pass1<T, RH>), grid, dim3(256), lds, s, a);
+    hipLaunchKernelGGL((k_pass1<T, RH, COL>), grid, dim3(256), lds, s, a);
     return hipGetLastError();
+}
+
+template <typename T, int RH>
+static hipError_t launch_pass1_t(const Pass1Args& a, hipStream_t s) {
+    const bool slots = pass1_slot_winners(a);
+    const size_t lds = pass1_lds_bytes((int)a.window, (int)sizeof(T), a.best != nullptr, slots, a.norm_sum != nullptr);
+    if (a.best != nullptr && !slots) return launch_pass1_k<T, RH, true>(a, lds, s);
+    return launch_pass1_k<T, RH, false>(a, lds, s);
 }
 
 hipError_t launch_pass1(const Pass1Args& a, hipStream_t s) {
