@@ -401,6 +401,21 @@ def test_graph_replays_equal_eager(ctx, fg, method):
     s.close()
 
 
+def test_hybrid_host_frames_several_launches(ctx, fg, oracle):
+    """Hybrid fed from host buffers: pass 1 runs once per 8 uploaded frames, and each launch's
+    winners (kept as frame slots of the cache) are resolved against the hard image of the earlier
+    launches."""
+    from oracle import make_pose, METHOD_HYBRID
+    spec, frames, poses = _synthetic(dict(frames=11, width=160, height=120))
+    opo = [make_pose(**p) for p in poses]
+    roi = oracle.common_overlap_roi([(spec.width, spec.height)] * spec.frames, opo)
+    s, out = _run_stack(fg, ctx, frames, poses, roi, method=fg.METHOD_HYBRID, debug_frame=1)
+    ref, dbg = oracle.stack(frames, opo, roi, method=METHOD_HYBRID, debug=True, frame_index=1)
+    _img_close(out, ref)
+    assert (s.readback("hard") != dbg["hard"]).mean() < 1e-3
+    s.close()
+
+
 @pytest.mark.parametrize("levels", [3, 4, 6, 7])
 def test_hybrid_levels_cap(ctx, fg, oracle, levels):
     """Hybrid with the pyramid capped: at 6 levels or fewer the hard range [2, 6) reaches the top
@@ -465,6 +480,7 @@ def test_errors_surface_as_status_codes(ctx, fg):
 
 SINGLE_CASES = {
     "hardmask_u8": ("hardmask", dict(frames=5, width=150, height=110)),
+    "hardmask_groups_u8": ("hardmask", dict(frames=11, width=97, height=70)),   # winners carried across launches
     "softmask_u8": ("softmask", dict(frames=5, width=150, height=110)),
     "softmask_groups_u8": ("softmask", dict(frames=11, width=97, height=70)),   # more frames than one group
     "softmask_u16": ("softmask", dict(frames=3, width=120, height=90, dtype="u16")),
