@@ -939,13 +939,12 @@ static int finish_tail(fg_stack* s) {
     if (s->accDbg)
         HIP_TRY(hipMemcpyAsync(s->accDbg, s->acc, s->lt.total * 3 * sizeof(float), hipMemcpyDeviceToDevice, s->stream),
                 "debug copy");
-    HIP_TRY(fg::launch_collapse(s->acc, &s->lt, s->stream), "collapse");
     const bool hybrid = s->opts.method == FG_METHOD_HYBRID_PYRAMID_CONTRAST;
     if (!hybrid) {
-        HIP_TRY(fg::launch_quantize(s->acc, s->W, s->H, s->sb, s->out_dev, s->stream), "quantize");
+        HIP_TRY(fg::launch_collapse_quantize(s->acc, &s->lt, s->sb, s->out_dev, s->stream), "collapse");
     } else {
         // src/stack/pipeline.zig:177-236: soft = the pyramid result, hard = winners, then merge
-        HIP_TRY(fg::launch_quantize(s->acc, s->W, s->H, s->sb, s->soft, s->stream), "quantize");
+        HIP_TRY(fg::launch_collapse_quantize(s->acc, &s->lt, s->sb, s->soft, s->stream), "collapse");
         HIP_TRY(fg::launch_hybrid_merge(s->soft, s->hard, s->sb, &s->lt, 2, 4, s->opts.hybrid_sharpness, s->merge_gs,
                                         s->merge_gh, s->acc, s->out_dev, s->stream),
                 "hybrid merge");

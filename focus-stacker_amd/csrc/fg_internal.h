@@ -105,8 +105,9 @@ size_t pass2_contrib_floats(const LevelTable* lt, uint32_t frames);
 // level-0 gaussian + mask of slot j, encoded as the level buffers are
 hipError_t launch_level0_encode(const Pass2Args& a, int slot, float4* dst, hipStream_t s);
 
-hipError_t launch_collapse(float* acc, const LevelTable* lt, hipStream_t s);
 hipError_t launch_quantize(const float* acc0, uint32_t w, uint32_t h, int sample_bytes, void* out, hipStream_t s);
+// collapse with its last step fused with the quantize (the collapsed level 0 is not kept)
+hipError_t launch_collapse_quantize(float* acc, const LevelTable* lt, int sample_bytes, void* out, hipStream_t s);
 hipError_t launch_zero(void* p, size_t bytes, hipStream_t s);
 hipError_t launch_fill_f32(float* p, size_t n, float v, hipStream_t s);
 

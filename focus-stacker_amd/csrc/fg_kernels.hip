@@ -68,17 +68,6 @@ __global__ void __launch_bounds__(256) k_collapse(float* __restrict__ parent, in
     parent[i + 2 * n] = parent[i + 2 * n] + e.z;
 }
 
-hipError_t launch_collapse(float* acc, const LevelTable* lt, hipStream_t s) {
-    for (int l = lt->n - 1; l >= 1; l--) {
-        const Level& p = lt->lv[l - 1];
-        const Level& c = lt->lv[l];
-        size_t n = (size_t)p.w * p.h;
-        hipLaunchKernelGGL(k_collapse, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, acc + 3 * p.off, (int)p.w,
-                           (int)p.h, (const float*)(acc + 3 * c.off), (int)c.w, (int)c.h);
-    }
-    return hipGetLastError();
-}
-
 template <typename T>
 __global__ void k_quantize(const float* __restrict__ acc0, size_t n, T* __restrict__ out) {
     const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
@@ -98,6 +87,48 @@ hipError_t launch_quantize(const float* acc0, uint32_t w, uint32_t h, int sb, vo
         hipLaunchKernelGGL((k_quantize<uint8_t>), grid, dim3(256), 0, s, acc0, n, (uint8_t*)out);
     else
         hipLaunchKernelGGL((k_quantize<uint16_t>), grid, dim3(256), 0, s, acc0, n, (uint16_t*)out);
+    return hipGetLastError();
+}
+
+// The last collapse step fused with the quantize: level 0 + EXPAND(level 1), then (int)clamp(v + 0.5)
+// (src/fuse/pyramid.zig:223-269), the same f32 operations without storing the collapsed level 0.
+template <typename T>
+__global__ void __launch_bounds__(256) k_collapse_quantize(const float* __restrict__ parent, int pw, int ph,
+                                                           const float* __restrict__ child, int cw, int ch,
+                                                           T* __restrict__ out) {
+    const size_t n = (size_t)pw * ph;
+    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const int x = (int)(i % pw), y = (int)(i / pw);
+    const f3 e = expand_at(PlanarAcc{child, (size_t)cw * ch, cw}, cw, ch, pw, ph, x, y);
+    const float v[3] = {parent[i] + e.x, parent[i + n] + e.y, parent[i + 2 * n] + e.z};
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        float q = v[c] + 0.5f;
+        q = q < 0.0f ? 0.0f : (q > SampleMax<T>::f ? SampleMax<T>::f : q);
+        out[i * 3 + c] = (T)(uint32_t)q;
+    }
+}
+
+hipError_t launch_collapse_quantize(float* acc, const LevelTable* lt, int sb, void* out, hipStream_t s) {
+    const Level& l0 = lt->lv[0];
+    if (lt->n == 1) return launch_quantize(acc, l0.w, l0.h, sb, out, s);
+    for (int l = lt->n - 1; l >= 2; l--) {
+        const Level& p = lt->lv[l - 1];
+        const Level& c = lt->lv[l];
+        size_t n = (size_t)p.w * p.h;
+        hipLaunchKernelGGL(k_collapse, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, acc + 3 * p.off, (int)p.w,
+                           (int)p.h, (const float*)(acc + 3 * c.off), (int)c.w, (int)c.h);
+    }
+    const Level& c = lt->lv[1];
+    const size_t n = (size_t)l0.w * l0.h;
+    const dim3 g((unsigned)((n + 255) / 256));
+    if (sb == 1)
+        hipLaunchKernelGGL((k_collapse_quantize<uint8_t>), g, dim3(256), 0, s, (const float*)(acc + 3 * l0.off),
+                           (int)l0.w, (int)l0.h, (const float*)(acc + 3 * c.off), (int)c.w, (int)c.h, (uint8_t*)out);
+    else
+        hipLaunchKernelGGL((k_collapse_quantize<uint16_t>), g, dim3(256), 0, s, (const float*)(acc + 3 * l0.off),
+                           (int)l0.w, (int)l0.h, (const float*)(acc + 3 * c.off), (int)c.w, (int)c.h, (uint16_t*)out);
     return hipGetLastError();
 }
 
@@ -241,9 +272,7 @@ hipError_t launch_hybrid_merge(const void* soft, const void* hard, int sb, const
                            (int)nx.w, (int)nx.h, top, (int)((uint32_t)l >= hs && (uint32_t)l < he), mix,
                            out_pyr + 3 * c.off);
     }
-    hipError_t e = launch_collapse(out_pyr, lt, s);
-    if (e != hipSuccess) return e;
-    return launch_quantize(out_pyr, l0.w, l0.h, sb, out, s);
+    return launch_collapse_quantize(out_pyr, lt, sb, out, s);
 }
 
 }  // namespace fg
