@@ -203,13 +203,20 @@ def test_stack_matches_oracle(ctx, fg, oracle, case):
     s.close()
 
 
-def test_stack_lens_poses(ctx, fg, oracle):
-    spec, frames, poses = _synthetic(dict(frames=4, width=150, height=110), lens=True)
+@pytest.mark.parametrize("dtype,method", [("u8", "pyramid"), ("u16", "pyramid"), ("u8", "hybrid"),
+                                          ("u16", "softmask")])
+def test_stack_lens_poses(ctx, fg, oracle, dtype, method):
+    """Real solved stacks carry radial terms and centre shifts: the lens remap path (Newton
+    undistortion, src/remap.zig:755-786) under every pass-1 variant."""
+    spec, frames, poses = _synthetic(dict(frames=4, width=150, height=110, dtype=dtype), lens=True)
     from oracle import make_pose
+    import oracle as om
+    fm = {"pyramid": fg.METHOD_PYRAMID, "hybrid": fg.METHOD_HYBRID, "softmask": fg.METHOD_SOFTMASK}[method]
+    o_m = {"pyramid": om.METHOD_PYRAMID, "hybrid": om.METHOD_HYBRID, "softmask": om.METHOD_SOFTMASK}[method]
     opo = [make_pose(**p) for p in poses]
     roi = oracle.common_overlap_roi([(spec.width, spec.height)] * spec.frames, opo)
-    s, out = _run_stack(fg, ctx, frames, poses, roi)
-    _img_close(out, oracle.stack(frames, opo, roi))
+    s, out = _run_stack(fg, ctx, frames, poses, roi, method=fm)
+    _img_close(out, oracle.stack(frames, opo, roi, method=o_m))
     s.close()
 
 
