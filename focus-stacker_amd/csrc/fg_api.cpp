@@ -683,7 +683,7 @@ static StackPlan plan_stack(const fg_stack_opts& o, double budget) {
     size_t fixed = pl.px * 4 + pl.px + pl.tot * 12 + pl.contrib_floats * 4 + pl.px * 3 * sb +
                    o.frame_count * sizeof(fg::FrameDesc);
     if (pl.hybrid || pl.hardmask) fixed += pl.px * 4 + pl.px * 3 * sb;
-    if (pl.hybrid) fixed += pl.px * 3 * sb + 2 * pl.tot * 12;
+    if (pl.hybrid) fixed += pl.px * 3 * sb + 2 * (pl.tot > pl.px ? pl.tot - pl.px : 1) * 12;
     if (o.debug_frame >= 0) fixed += pl.tot * 12;
     const size_t per_frame = pl.px * 4 * sb + pl.px * 4 + pl.lev_stride * sizeof(float4);
     pl.cached = !pl.single && (double)fixed + (double)o.frame_count * (double)per_frame <= budget;
@@ -784,8 +784,10 @@ int fg_stack_create_budget(fg_context* ctx, const fg_stack_opts* o, double budge
     }
     if (pl.hybrid) {
         ALLOC(soft, px * 3 * s->sb);
-        ALLOC(merge_gs, tot * 3 * sizeof(float));
-        ALLOC(merge_gh, tot * 3 * sizeof(float));
+        // gaussian levels 1.. of the merge; level 0 is read from the images
+        const size_t up = tot > px ? tot - px : 1;
+        ALLOC(merge_gs, up * 3 * sizeof(float));
+        ALLOC(merge_gh, up * 3 * sizeof(float));
     }
     if (o->debug_frame >= 0) ALLOC(accDbg, tot * 3 * sizeof(float));
 #undef ALLOC

@@ -126,7 +126,9 @@ hipError_t launch_soft(const SoftArgs& a, hipStream_t s);
 hipError_t launch_soft_finalize(const float* wsum, const float* psum, uint32_t w, uint32_t h, int sample_bytes,
                                 void* out, hipStream_t s);
 
-// hybrid merge (mergeHybridImages): Laplacian pyramids of soft and hard, mix, collapse, quantize
+// hybrid merge (mergeHybridImages): Laplacian pyramids of soft and hard, mix, collapse, quantize.
+// gs and gh hold gaussian levels 1..n-1 only (level l at 3 * (off_l - off_1) floats; level 0 is
+// read from the images); out_pyr is laid out like the accumulator.
 hipError_t launch_hybrid_merge(const void* soft, const void* hard, int sample_bytes, const LevelTable* lt,
                                uint32_t hard_start, uint32_t hard_count, float mix, float* gs, float* gh,
                                float* out_pyr, void* out, hipStream_t s);
