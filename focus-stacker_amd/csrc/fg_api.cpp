@@ -1063,6 +1063,11 @@ int fg_stack_debug_readback(fg_stack* s, int which, uint32_t index, void* dst, s
             const fg::Level& l = s->lt.lv[index];
             const size_t n = (size_t)l.w * l.h;
             if (which == FG_DEBUG_ACC_LEVEL) {
+                if (!s->finished && (s->cached || s->pass2_done == 0)) {  // no frame accumulated yet
+                    if (n * 12 > dst_bytes) return fail(FG_ERR_INVALID_ARG, "destination too small");
+                    memset(dst, 0, n * 12);
+                    return FG_OK;
+                }
                 const float* src = s->acc;
                 if (s->finished) {
                     if (!s->accDbg) return fail(FG_ERR_STATE, "accumulator is collapsed in place by finish");
@@ -1124,7 +1129,8 @@ int fg_stack_reset(fg_stack* s) {
     const size_t px = (size_t)s->W * s->H;
     HIP_TRY(fg::launch_zero(s->S, px * sizeof(float), s->stream), "reset");
     HIP_TRY(fg::launch_zero(s->U, px, s->stream), "reset");
-    HIP_TRY(fg::launch_zero(s->acc, s->lt.total * 3 * sizeof(float), s->stream), "reset");
+    // pass 2 starts its sums at frame 0 without reading the accumulator; softmask adds into it
+    if (s->single) HIP_TRY(fg::launch_zero(s->acc, s->lt.total * 3 * sizeof(float), s->stream), "reset");
     if (s->best) HIP_TRY(fg::launch_fill_f32(s->best, px, -INFINITY, s->stream), "reset");
     s->added = 0;
     s->p1_done = 0;

@@ -923,12 +923,13 @@ __global__ void __launch_bounds__(256) k_pass2(Pass2Args a, LevelArgs g) {
     const bool inner = clamp_regime && !g.top && X0 >= 4 && Y0 >= 4 && 2 * (X0 / 2 + TW / 2 + 2) <= 2 * g.nw - 2 &&
                        2 * (Y0 / 2 + TH / 2 + 2) <= 2 * g.nh - 2 && X0 + TW <= g.w && Y0 + TH <= g.h;
 
+    // a launch that starts at frame 0 starts its sums from 0.0f, as the accumulator would hold
     float ar[TL::PPT], ag[TL::PPT], ab[TL::PPT];
 #pragma unroll
     for (int j = 0; j < TL::PPT; j++) {
         ar[j] = ag[j] = ab[j] = 0.0f;
         const int x = X0 + tx, y = Y0 + ty + j * TL::STEP;
-        if (!SPLIT && x < g.w && y < g.h) {
+        if (!SPLIT && a.k0 > 0 && x < g.w && y < g.h) {
             const int i = y * g.w + x;
             ar[j] = accp[i];
             ag[j] = accp[i + n];
@@ -1238,12 +1239,12 @@ __global__ void __launch_bounds__(256) k_pass2(Pass2Args a, LevelArgs g) {
     }
 }
 
-// acc += contribution of each frame, in frame order
+// acc += contribution of each frame, in frame order (from 0.0f when the frames start at frame 0)
 __global__ void __launch_bounds__(256) k_ordered_sum(float* __restrict__ acc, const float* __restrict__ contrib,
-                                                     int nk, size_t n3) {
+                                                     int nk, size_t n3, int fresh) {
     const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
     if (i >= n3) return;
-    float v = acc[i];
+    float v = fresh ? 0.0f : acc[i];
     for (int k = 0; k < nk; k++) v = v + contrib[(size_t)k * n3 + i];
     acc[i] = v;
 }
@@ -1277,7 +1278,7 @@ static hipError_t launch_level(const Pass2Args& a, const LevelArgs& g, int nk, h
         dim3 grid((g.w + 15) / 16, (g.h + 15) / 16, nk);
         hipLaunchKernelGGL((k_pass2<T, L0, 16, 16, true>), grid, dim3(256), 0, s, a, g);
         hipLaunchKernelGGL(k_ordered_sum, dim3((unsigned)((n3 + 255) / 256)), dim3(256), 0, s, a.acc + g.acc_off,
-                           (const float*)a.contrib, nk, n3);
+                           (const float*)a.contrib, nk, n3, (int)(a.k0 == 0));
     }
     return hipGetLastError();
 }

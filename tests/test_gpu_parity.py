@@ -453,6 +453,21 @@ def test_uncached_second_pass_equals_cached(ctx, fg, oracle, method):
     assert np.array_equal(out1, out2)
     if method == "hybrid":
         assert np.array_equal(s1.readback("hard"), s2.readback("hard"))
+    # reset and reused with another frame set: pass 2 starts its sums at frame 0 without reading
+    # the accumulator (reset leaves it as the last finish did), so a stale sum would show here
+    flipped = [np.ascontiguousarray(f[::-1, ::-1]) for f in frames[::-1]]
+    s3, fresh = _run_stack(fg, ctx, flipped, poses, roi, method=m)
+    s3.close()
+    assert not np.array_equal(fresh, out1)
+    for st in (s1, s2):
+        st.reset()
+        assert not st.readback("acc_level", 0).any()     # nothing accumulated yet reads as zeros
+        for f, p in zip(flipped, poses):
+            st.add_frame(f, fg.pose(p), roi)
+        if st.needs_pass2_frames():
+            for f, p in zip(flipped, poses):
+                st.pass2_frame(f, fg.pose(p), roi)
+        assert np.array_equal(st.finish(), fresh)
     s1.close()
     s2.close()
 
