@@ -45,7 +45,7 @@ def kernel_models(s: int) -> dict:
 
 # HBM bytes per launch of the default workload, from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE
 # runs scaled by the same counters on a plain copy (profiles/pmc_traffic_30x7952x5304_u8.md)
-TRAFFIC = {"k_pass1": 20.263e9, "k_pass2_level0": 29.720e9}
+TRAFFIC = {"k_pass1": 20.263e9, "k_pass2_level0": 28.988e9}
 COPY_PROBE_BYTES = 256 << 20
 
 
