@@ -510,6 +510,56 @@ SINGLE_CASES = {
 }
 
 
+def _sweep_case(seed):
+    """A seeded random stack: size, frame count, sample type, method, contrast window, pose
+    magnitudes and the lens path drawn together, so the sweep crosses combinations that the
+    hand-picked cases above do not."""
+    rng = np.random.default_rng(90210 + seed)
+    small = rng.random() < 0.3
+    w = int(rng.integers(2, 40)) if small else int(rng.integers(40, 421))   # the generator needs 2+
+    h = int(rng.integers(2, 40)) if small else int(rng.integers(40, 421))
+    return dict(seed=seed, w=w, h=h, frames=int(rng.integers(1, 7)),
+                dtype=("u8", "u16")[(seed // 4) % 2],
+                method=("pyramid", "hybrid", "hardmask", "softmask")[seed % 4],
+                window=int(rng.choice([3, 5, 5, 7, 9, 11])), lens=bool(rng.random() < 0.3),
+                shift=float(rng.uniform(0.0, min(3.0, max(w, h) / 10.0))),
+                roll=float(rng.uniform(0.0, 0.3)), scale=float(rng.uniform(0.0, 0.01)))
+
+
+@pytest.mark.parametrize("seed", range(24))
+def test_random_stacks_match_oracle(ctx, fg, oracle, seed):
+    """Seeded sweep over whole stacks through the C ABI, each against the oracle with the same
+    options; the case is printed first so a failure names it."""
+    import synth
+    import oracle as om
+    c = _sweep_case(seed)
+    print(f"sweep case {c}", flush=True)
+    spec = synth.config_spec(0, stack_id=1000 + seed, frames=c["frames"], width=c["w"], height=c["h"],
+                             dtype=c["dtype"], lens=c["lens"], shift=c["shift"], roll_deg=c["roll"],
+                             scale=c["scale"], sigma_max=3.0)
+    st = synth.Stack(spec)
+    frames = [st.frame_numpy(k) for k in range(spec.frames)]
+    poses = st.poses
+    from oracle import make_pose
+    opo = [make_pose(**p) for p in poses]
+    roi = oracle.common_overlap_roi([(spec.width, spec.height)] * spec.frames, opo)
+    if roi[2] - roi[0] < 1 or roi[3] - roi[1] < 1:
+        roi = (0, 0, spec.width, spec.height)
+    fm = {"pyramid": fg.METHOD_PYRAMID, "hybrid": fg.METHOD_HYBRID, "hardmask": fg.METHOD_HARDMASK,
+          "softmask": fg.METHOD_SOFTMASK}[c["method"]]
+    o_m = {"pyramid": om.METHOD_PYRAMID, "hybrid": om.METHOD_HYBRID, "hardmask": om.METHOD_HARDMASK,
+           "softmask": om.METHOD_SOFTMASK}[c["method"]]
+    s, out = _run_stack(fg, ctx, frames, poses, roi, method=fm, window=c["window"])
+    s.close()
+    ref = oracle.stack(frames, opo, roi, method=o_m, window=c["window"])
+    assert out.shape == ref.shape
+    if c["method"] == "hardmask":
+        differ = int((out != ref).any(axis=2).sum())
+        assert differ <= max(1, int(1e-3 * out.shape[0] * out.shape[1])), f"{differ} winners differ"
+    else:
+        _img_close(out, ref)
+
+
 @pytest.mark.parametrize("case", list(SINGLE_CASES))
 def test_single_pass_methods_match_oracle(ctx, fg, oracle, case):
     """hardmask-contrast and softmask-contrast (src/stack/pipeline.zig:245-312)."""
