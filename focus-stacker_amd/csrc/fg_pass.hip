@@ -88,17 +88,19 @@ constexpr int P1C = 64;
 
 int pass1_rows(int window) { return window / 2 <= 4 ? 32 : 64; }
 
-// dynamic LDS of k_pass1 (byte offsets of every array in Pass1Lds): region gray and alpha, the
-// tile's running norm sum and union support (not for the single-pass methods) and, for the hybrid
-// and hardmask methods, the winning weight and either the winner's frame slot (frames cached: its
-// colour is read back from the frame cache at the end) or the region colour (frames not cached:
-// a new winner's colour is written to the hard image right away)
+// dynamic LDS of k_pass1 (byte offsets of every array in Pass1Lds): region gray and alpha (one
+// byte per pixel for u8 frames), the tile's running norm sum and union support (a byte per pixel;
+// for the u8 hybrid one 64-bit lane mask per tile row; neither for the single-pass methods) and, for
+// the hybrid and hardmask methods, the winning weight and either the winner's frame slot (frames
+// cached: its colour is read back from the frame cache at the end) or the region colour (frames
+// not cached: a new winner's colour is written to the hard image right away).  At u8 and windows
+// 5 to 9 the cached hybrid's arrays fit 40 KB, four blocks per CU.
 struct Pass1Lds {
     size_t gray, alpha, colour, stage, tS, tU, tB, tK, total;
 };
 // staged source box: 4 waves x StageGeom::NR rows of 64 x StageGeom::NC dwords
 constexpr size_t kStageBytesU8 = 4 * 12 * 64 * 1 * 4, kStageBytesU16 = 4 * 10 * 64 * 2 * 4;
-__host__ __device__ inline Pass1Lds pass1_lds(int rh, int th, int sb, bool hybrid, bool cached, bool acc) {
+__host__ __device__ inline Pass1Lds pass1_lds(int rh, int th, int sb, bool hybrid, bool cached, bool acc, bool pack) {
     Pass1Lds o;
     const size_t reg = (size_t)rh * P1C, til = (size_t)th * P1C, pt = sb == 1 ? 4 : 8;
     const bool colour = hybrid && !cached;
@@ -109,8 +111,8 @@ __host__ __device__ inline Pass1Lds pass1_lds(int rh, int th, int sb, bool hybri
     o.tB = at;     at += hybrid ? til * 4 : 0;
     o.colour = at; at += colour ? reg * pt : 0;
     o.tK = at;     at += hybrid && cached ? til * 2 : 0;
-    o.alpha = at;  at += reg * 2;
-    o.tU = at;     at += acc ? til : 0;
+    o.alpha = at;  at += reg * (pack ? 1 : 2);
+    o.tU = at;     at += acc ? (pack ? (size_t)th * 8 : til) : 0;
     o.total = (at + 15) & ~(size_t)15;
     return o;
 }
@@ -121,9 +123,9 @@ __host__ __device__ inline bool pass1_slot_winners(const Pass1Args& a) {
     return a.rgba != nullptr && a.k1 - a.slot_base < 0xffff;
 }
 
-size_t pass1_lds_bytes(int window, int sb, bool hybrid, bool cached, bool acc) {
+size_t pass1_lds_bytes(int window, int sb, bool hybrid, bool cached, bool acc, bool pack) {
     const int b = window / 2, rh = pass1_rows(window);
-    return pass1_lds(rh, rh - 2 * b, sb, hybrid, cached, acc).total;
+    return pass1_lds(rh, rh - 2 * b, sb, hybrid, cached, acc, pack).total;
 }
 
 // 1 / d to within a few ulp: hardware estimate and two Newton steps.
@@ -454,7 +456,9 @@ __device__ __forceinline__ V lane_window_sum(V x, int b) {
 
 // COL: hybrid or hardmask winners without a frame cache (the region colour in LDS, new winners
 // written to the hard image at once); compiled apart so that the other launches carry none of it.
-template <typename T, int RH, bool COL>
+// PACK: the u8 hybrid with its frames cached, whose LDS is packed (alpha bytes, union support as
+// row lane masks) to keep four blocks per CU; likewise compiled apart.
+template <typename T, int RH, bool COL, bool PACK>
 __global__ void __launch_bounds__(256, sizeof(T) == 1 ? 4 : 3) k_pass1(Pass1Args a) {
     using P = Pack<T>;
     using PT = typename P::type;
@@ -468,13 +472,18 @@ __global__ void __launch_bounds__(256, sizeof(T) == 1 ? 4 : 3) k_pass1(Pass1Args
     // hybrid with the frames cached: the winner is kept as its frame slot, its colour read back
     // from the frame cache when the tile is written (the RGBA the region colour would have held)
     const bool hyb_slot = !COL && hyb;  // the launch picks COL whenever there are winners and no cache
-    const Pass1Lds L = pass1_lds(RH, th, (int)sizeof(T), hyb, pass1_slot_winners(a), a.norm_sum != nullptr);
+    const Pass1Lds L = pass1_lds(RH, th, (int)sizeof(T), hyb, pass1_slot_winners(a), a.norm_sum != nullptr, PACK);
     float* gray = (float*)(smem + L.gray);            // [RH][64] gray, -1 where alpha == 0 or outside
-    uint16_t* alph = (uint16_t*)(smem + L.alpha);     // [RH][64] alpha
+    using AT = std::conditional_t<PACK, uint8_t, uint16_t>;
+    AT* alph = (AT*)(smem + L.alpha);                 // [RH][64] alpha
     PT* colour = (PT*)(smem + L.colour);              // [RH][64] hybrid: packed rgb
     uint32_t* stage = (uint32_t*)(smem + L.stage);    // staged source box of the current frame
     float* tS = (float*)(smem + L.tS);                // [th][64] norm sum of the tile
-    uint8_t* tU = smem + L.tU;                        // [th][64] union support
+    // union support of the tile: a byte per pixel, or for the u8 hybrid (whose winner arrays fill
+    // its LDS) one 64-bit lane mask per tile row, which keeps it to four blocks per CU
+    const bool umask = PACK && acc && hyb;
+    uint8_t* tUb = smem + L.tU;                       // [th][64]
+    uint64_t* tUw = (uint64_t*)(smem + L.tU);         // [th]
     float* tB = (float*)(smem + L.tB);                // [th][64] hybrid: best weight
     uint16_t* tK = (uint16_t*)(smem + L.tK);          // [th][64] hybrid, frames cached: its slot
     __shared__ double corners[8];                     // next frame's region corners in its source
@@ -530,7 +539,12 @@ __global__ void __launch_bounds__(256, sizeof(T) == 1 ? 4 : 3) k_pass1(Pass1Args
         }
         if (acc) {  // the single-pass methods have no norm sum: tS and tU take no LDS
             tS[r * P1C + lane] = sv;
-            tU[r * P1C + lane] = (uint8_t)uv;
+            if (umask) {
+                const uint64_t um = __ballot(uv != 0);  // every lane of the wave is here
+                if (lane == 0) tUw[r] = um;
+            } else {
+                tUb[r * P1C + lane] = (uint8_t)uv;
+            }
         }
         if (hyb) tB[r * P1C + lane] = bv;
         if (hyb_slot) tK[r * P1C + lane] = 0xffffu;   // no winner in this launch yet
@@ -592,7 +606,7 @@ __global__ void __launch_bounds__(256, sizeof(T) == 1 ? 4 : 3) k_pass1(Pass1Args
                         ((uint2*)rgba)[i] = *(const uint2*)&c;
                 }
                 gray[r * P1C + lane] = g;
-                alph[r * P1C + lane] = (uint16_t)o[3];
+                alph[r * P1C + lane] = (AT)o[3];
                 if (COL) colour[r * P1C + lane] = c;
             }
         };
@@ -646,6 +660,10 @@ __global__ void __launch_bounds__(256, sizeof(T) == 1 ? 4 : 3) k_pass1(Pass1Args
                 n = lane_window_sum(vn, b);
             }
             const int y = Y0 + r;
+            if (umask) {  // accumulateBinarySupportMax as one lane-mask OR per row (hybrid layout)
+                const uint64_t um = __ballot(col_ok && y < H && alph[(r + b) * P1C + lane] > 0);
+                if (lane == 0) atomicOr((unsigned long long*)&tUw[r], (unsigned long long)um);
+            }
             if (!col_ok || y >= H) continue;
             const size_t i = (size_t)y * W + ox;
             float stdw = 0.0f;
@@ -672,7 +690,7 @@ __global__ void __launch_bounds__(256, sizeof(T) == 1 ? 4 : 3) k_pass1(Pass1Args
             const int t = r * P1C + lane;
             if (acc) {
                 tS[t] = tS[t] + wv;
-                if (alpha > 0) tU[t] = 1;
+                if (!umask && alpha > 0) tUb[t] = 1;
             }
             if (hyb && wv > tB[t]) {  // updateRange: strictly greater wins, first frame on ties
                 tB[t] = wv;
@@ -705,7 +723,7 @@ __global__ void __launch_bounds__(256, sizeof(T) == 1 ? 4 : 3) k_pass1(Pass1Args
         const int t = r * P1C + lane;
         if (acc) {
             a.norm_sum[i] = tS[t];
-            a.union_sup[i] = (uint8_t)tU[t];
+            a.union_sup[i] = umask ? (uint8_t)((tUw[r] >> lane) & 1u) : tUb[t];
         }
         if (hyb) a.best[i] = tB[t];
         if (hyb_slot) {
@@ -721,25 +739,30 @@ __global__ void __launch_bounds__(256, sizeof(T) == 1 ? 4 : 3) k_pass1(Pass1Args
     }
 }
 
-template <typename T, int RH, bool COL>
-static hipError_t launch_pass1_k(const Pass1Args& a, size_t lds, hipStream_t s) {
+template <typename T, int RH, bool COL, bool PACK>
+static hipError_t launch_pass1_k(const Pass1Args& a, hipStream_t s) {
+    const size_t lds = pass1_lds_bytes((int)a.window, (int)sizeof(T), a.best != nullptr, pass1_slot_winners(a),
+                                       a.norm_sum != nullptr, PACK);
     const int b = (int)a.window / 2, th = RH - 2 * b, tw = P1C - 2 * b;
     if (lds > 65536) {
         hipError_t e =
-            hipFuncSetAttribute((const void*)k_pass1<T, RH, COL>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+            hipFuncSetAttribute((const void*)k_pass1<T, RH, COL, PACK>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)lds);
         if (e != hipSuccess) return e;
     }
     dim3 grid((a.w + tw - 1) / tw, (a.h + th - 1) / th);
-    hipLaunchKernelGGL((k_pass1<T, RH, COL>), grid, dim3(256), lds, s, a);
+    hipLaunchKernelGGL((k_pass1<T, RH, COL, PACK>), grid, dim3(256), lds, s, a);
     return hipGetLastError();
 }
 
 template <typename T, int RH>
 static hipError_t launch_pass1_t(const Pass1Args& a, hipStream_t s) {
     const bool slots = pass1_slot_winners(a);
-    const size_t lds = pass1_lds_bytes((int)a.window, (int)sizeof(T), a.best != nullptr, slots, a.norm_sum != nullptr);
-    if (a.best != nullptr && !slots) return launch_pass1_k<T, RH, true>(a, lds, s);
-    return launch_pass1_k<T, RH, false>(a, lds, s);
+    if (a.best != nullptr && !slots) return launch_pass1_k<T, RH, true, false>(a, s);
+    if constexpr (sizeof(T) == 1) {
+        if (a.best != nullptr && a.norm_sum != nullptr) return launch_pass1_k<T, RH, false, true>(a, s);
+    }
+    return launch_pass1_k<T, RH, false, false>(a, s);
 }
 
 hipError_t launch_pass1(const Pass1Args& a, hipStream_t s) {
