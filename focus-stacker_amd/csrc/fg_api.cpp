@@ -1,4 +1,4 @@
-// C ABI of libfocus_gpu.so (include/focus_gpu.h): contexts, the stack driver (pass 1 / pass 2 /
+// C ABI of libfocus_gpu.so (include/focus_gpu.h): contexts, the stack sequence (pass 1 / pass 2 /
 // collapse, src/stack/pipeline.zig:361-605 and src/fuse/pipeline.zig:254-445), the frame cache
 // under the memory budget, and the host-side pose preparation (src/optimize.zig:3555-3586).
 #include <hip/hip_runtime.h>
@@ -163,7 +163,7 @@ struct fg_context {
     hipStream_t stream;
 };
 
-// One stack: the pass-1 / pass-2 driver.  Pass 1 runs over groups of frames (all pending frames
+// One stack: the pass-1 / pass-2 sequence.  Pass 1 runs over groups of frames (all pending frames
 // at finish, or whenever the host staging slots are full); pass 2 runs over every cached frame
 // at finish, or frame by frame when the frames are fed a second time.
 struct fg_stack {
