@@ -1063,7 +1063,7 @@ int fg_stack_debug_readback(fg_stack* s, int which, uint32_t index, void* dst, s
             const fg::Level& l = s->lt.lv[index];
             const size_t n = (size_t)l.w * l.h;
             if (which == FG_DEBUG_ACC_LEVEL) {
-                if (!s->finished && (s->cached || s->pass2_done == 0)) {  // no frame accumulated yet
+                if (!s->single && !s->finished && (s->cached || s->pass2_done == 0)) {  // nothing summed yet
                     if (n * 12 > dst_bytes) return fail(FG_ERR_INVALID_ARG, "destination too small");
                     memset(dst, 0, n * 12);
                     return FG_OK;
