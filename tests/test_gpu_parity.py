@@ -526,7 +526,7 @@ def _sweep_case(seed):
                 roll=float(rng.uniform(0.0, 0.3)), scale=float(rng.uniform(0.0, 0.01)))
 
 
-@pytest.mark.parametrize("seed", range(24))
+@pytest.mark.parametrize("seed", range(48))
 def test_random_stacks_match_oracle(ctx, fg, oracle, seed):
     """Seeded sweep over whole stacks through the C ABI, each against the oracle with the same
     options; the case is printed first so a failure names it."""
