@@ -107,7 +107,9 @@ __device__ __forceinline__ TileAcc stage_window(float* t, const float* __restric
     return store_window(t, load_window(src, cw, ch));
 }
 
-static inline dim3 expand_grid(int w, int h) { return dim3((unsigned)((w + kExW - 1) / kExW), (unsigned)((h + kExH - 1) / kExH)); }
+static inline dim3 expand_grid(int w, int h) {
+    return dim3((unsigned)((w + kExW - 1) / kExW), (unsigned)((h + kExH - 1) / kExH));
+}
 
 __global__ void __launch_bounds__(256) k_collapse(float* __restrict__ parent, int pw, int ph,
                                                   const float* __restrict__ child, int cw, int ch) {
