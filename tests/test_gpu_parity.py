@@ -315,6 +315,29 @@ def test_tiny_sources_remapped(ctx, fg, oracle, size, dtype):
     _img_close(out, oracle.stack(frames, [make_pose(**p) for p in poses], roi))
 
 
+@pytest.mark.parametrize("levels", [0, 6])
+def test_cpu_runnable_config_matches_oracle(ctx, fg, oracle, levels):
+    """The reference's CPU-runnable case at its own shape: 8 seeded 1024x768 u8 frames under
+    similarity poses, remap + pyramid-contrast, at full depth (11 levels) and capped at the 6
+    levels its configuration names; the fused image and every intermediate level."""
+    spec, frames, poses = _synthetic(dict())
+    assert (spec.frames, spec.width, spec.height) == (8, 1024, 768)
+    from oracle import make_pose
+    opo = [make_pose(**p) for p in poses]
+    roi = oracle.common_overlap_roi([(spec.width, spec.height)] * spec.frames, opo)
+    s, out = _run_stack(fg, ctx, frames, poses, roi, max_levels=levels, debug_frame=3)
+    ref, dbg = oracle.stack(frames, opo, roi, max_levels=levels, debug=True, frame_index=3)
+    assert len(s.levels()) == (11 if levels == 0 else levels)
+    d = _img_close(out, ref)
+    print(f"cfg0 levels={levels} roi={roi}: max {d.max()} mean {d.mean():.2e}")
+    _level_close(s.readback("norm_sum"), dbg["norm_sum"], 1e-5)
+    for l in range(len(s.levels())):
+        _level_close(s.readback("acc_level", l), dbg["acc_levels"][l], 1e-5)
+        _level_close(s.readback("mask_level", l), dbg["mask_levels"][l], 1e-5)
+        _level_close(s.readback("gauss_level", l), dbg["gauss_levels"][l], 1e-5)
+    s.close()
+
+
 def test_levels_cap(ctx, fg, oracle):
     spec, frames, poses = _synthetic(dict(frames=3, width=128, height=96))
     from oracle import make_pose
