@@ -189,7 +189,8 @@ void* fg_stack_stream(const fg_stack* stack);
 int fg_stack_sync(fg_stack* stack);
 
 /* Starts a new stack of the same geometry and options on the same device buffers (no
- * allocation): clears the norm sum, union support, accumulators and frame counters. */
+ * allocation): clears the norm sum, union support and frame counters.  The accumulators start
+ * over with the first pass-2 frame (softmask's sums are cleared here). */
 int fg_stack_reset(fg_stack* stack);
 
 /* Device time per phase, from HIP events recorded on the stack's stream while profiling is
