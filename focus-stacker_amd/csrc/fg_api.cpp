@@ -161,6 +161,12 @@ struct fg_context {
     int device;
     float memory_fraction;
     hipStream_t stream;
+    // fg_remap's device copies of the source and the output: grown on demand, kept until the
+    // context ends (one hipFree per growth instead of a hipMalloc / hipFree pair per call)
+    std::mutex remap_mu;
+    void* remap_in;
+    void* remap_out;
+    size_t remap_in_cap, remap_out_cap;
 };
 
 // One stack: the pass-1 / pass-2 sequence.  Pass 1 runs over groups of frames (all pending frames
@@ -201,6 +207,7 @@ struct fg_stack {
     std::vector<uint8_t*> stage;  // device copies of host-fed source frames awaiting pass 1
     size_t stage_bytes;
     uint32_t stage_used;
+    uint32_t stage_slots;         // slots of the ring (kStageSlots, fewer if device memory ran short)
     // uploads of host-fed frames run on their own stream: the compute stream waits for a slot's
     // upload, the next upload into a slot waits until pass 1 has read it
     hipStream_t cstream;
@@ -259,7 +266,7 @@ struct PhaseMark {
 // Marks the staging slots [0, n) as read once the work queued so far on the compute stream is done.
 int release_slots(fg_stack* s, uint32_t n) {
     if (!s->cstream) return FG_OK;
-    for (uint32_t j = 0; j < n && j < kStageSlots; j++) {
+    for (uint32_t j = 0; j < n && j < s->stage_slots; j++) {
         HIP_TRY(hipEventRecord(s->free_ev[j], s->stream), "slot event");
         s->free_rec[j] = true;
     }
@@ -298,10 +305,13 @@ int dev_alloc(fg_stack* s, size_t bytes, void** out) {
     return FG_OK;
 }
 
-void free_stack(fg_stack* s) {
+// `sync`: wait for the stack's streams first; a caller that knows the stack's work is complete
+// (a pooled server stack, whose completion event fired) passes false, since the stream may be a
+// shared one that already runs another stack's work.
+void free_stack(fg_stack* s, bool sync = true) {
     if (!s) return;
-    if (s->cstream) (void)hipStreamSynchronize(s->cstream);
-    if (s->stream) (void)hipStreamSynchronize(s->stream);
+    if (sync && s->cstream) (void)hipStreamSynchronize(s->cstream);
+    if (sync && s->stream) (void)hipStreamSynchronize(s->stream);
     if (s->cstream) {
         for (int j = 0; j < 8; j++) {
             (void)hipEventDestroy(s->up_ev[j]);
@@ -391,12 +401,19 @@ int describe_frame(fg_stack* s, const fg_image_desc* d, const void* px, bool dev
                     s->free_rec[j] = false;
                 }
             }
+            // the ring is sized for overlap, not needed for correctness: when device memory runs
+            // short it keeps the slots it got (at least one), and pass 1 then runs more often
             for (uint32_t j = 0; j < kStageSlots; j++) {
                 void* p = nullptr;
                 rc = dev_alloc(s, nb, &p);
-                if (rc) return rc;
+                if (rc) {
+                    if (j == 0) return rc;
+                    g_last_error.clear();
+                    break;
+                }
                 s->stage.push_back((uint8_t*)p);
             }
+            s->stage_slots = (uint32_t)s->stage.size();
             s->stage_bytes = nb;
         }
         const uint32_t j = s->stage_used;
@@ -586,6 +603,8 @@ int fg_context_create(const fg_context_opts* opts, fg_context** out) {
     if (!c) return fail(FG_ERR_OOM_HOST, "context allocation");
     c->device = dev;
     c->memory_fraction = opts ? opts->memory_fraction : 0.5f;
+    c->remap_in = c->remap_out = nullptr;
+    c->remap_in_cap = c->remap_out_cap = 0;
     e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
     if (e != hipSuccess) {
         delete c;
@@ -600,6 +619,8 @@ void fg_context_destroy(fg_context* ctx) {
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
     (void)hipStreamDestroy(ctx->stream);
+    if (ctx->remap_in) (void)hipFree(ctx->remap_in);
+    if (ctx->remap_out) (void)hipFree(ctx->remap_out);
     delete ctx;
 }
 
@@ -634,25 +655,30 @@ int fg_remap(fg_context* ctx, const fg_image_desc* src, const void* src_px, cons
     size_t sb = src->sample_type == FG_SAMPLE_U16 ? 2 : 1;
     size_t in_b = desc_bytes(src);
     size_t out_b = (size_t)(r.right - r.left) * (r.bottom - r.top) * (src->color_channels + 1) * sb;
-    void *din = nullptr, *dout = nullptr;
-    hipError_t e = hipMalloc(&din, in_b);
+    // the context's scratch copies, grown when a larger frame comes (calls on one context are
+    // serialised here: they share the buffers and the context's stream)
+    std::lock_guard<std::mutex> lk(ctx->remap_mu);
+    auto grow = [&](void** buf, size_t* cap, size_t need) -> hipError_t {
+        if (*cap >= need) return hipSuccess;
+        if (*buf) (void)hipFree(*buf);
+        *buf = nullptr;
+        *cap = 0;
+        hipError_t e2 = hipMalloc(buf, need);
+        if (e2 == hipSuccess) *cap = need;
+        return e2;
+    };
+    hipError_t e = grow(&ctx->remap_in, &ctx->remap_in_cap, in_b);
+    if (e == hipSuccess) e = grow(&ctx->remap_out, &ctx->remap_out_cap, out_b);
     if (e != hipSuccess) return fail_hip(e, "hipMalloc");
-    e = hipMalloc(&dout, out_b);
-    if (e != hipSuccess) {
-        (void)hipFree(din);
-        return fail_hip(e, "hipMalloc");
-    }
     rc = FG_OK;
-    e = hipMemcpyAsync(din, src_px, in_b, hipMemcpyHostToDevice, ctx->stream);
+    e = hipMemcpyAsync(ctx->remap_in, src_px, in_b, hipMemcpyHostToDevice, ctx->stream);
     if (e == hipSuccess) {
-        rc = fg_remap_device(ctx, src, din, pose, &r, dout, ctx->stream);
+        rc = fg_remap_device(ctx, src, ctx->remap_in, pose, &r, ctx->remap_out, ctx->stream);
         if (rc == FG_OK) {
-            e = hipMemcpyAsync(dst_px, dout, out_b, hipMemcpyDeviceToHost, ctx->stream);
+            e = hipMemcpyAsync(dst_px, ctx->remap_out, out_b, hipMemcpyDeviceToHost, ctx->stream);
             if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
         }
     }
-    (void)hipFree(din);
-    (void)hipFree(dout);
     if (rc) return rc;
     if (e != hipSuccess) return fail_hip(e, "fg_remap");
     return FG_OK;
@@ -686,7 +712,11 @@ static StackPlan plan_stack(const fg_stack_opts& o, double budget) {
     if (pl.hybrid) fixed += pl.px * 3 * sb + 2 * (pl.tot > pl.px ? pl.tot - pl.px : 1) * 12;
     if (o.debug_frame >= 0) fixed += pl.tot * 12;
     const size_t per_frame = pl.px * 4 * sb + pl.px * 4 + pl.lev_stride * sizeof(float4);
-    pl.cached = !pl.single && (double)fixed + (double)o.frame_count * (double)per_frame <= budget;
+    // frames fed from host memory go through an upload ring of kStageSlots source frames, which
+    // describe_frame allocates at the first such frame; the caching decision leaves room for it
+    // (sources the size of the fused area; a larger source makes the ring shrink, not fail)
+    const double ring = (double)kStageSlots * (double)pl.px * 3.0 * (double)sb;
+    pl.cached = !pl.single && (double)fixed + ring + (double)o.frame_count * (double)per_frame <= budget;
     pl.slots = pl.cached ? o.frame_count : 1;
     // single-pass methods keep no frame past its group: softmask holds a group of frames
     // between pass 1 and the blend, hardmask only the winners
@@ -812,6 +842,18 @@ void fg_stack_destroy(fg_stack* s) {
     free_stack(s);
 }
 
+}  // extern "C"
+
+void fg_stack_destroy_idle(fg_stack* s) {
+    if (!s) return;
+    (void)hipSetDevice(s->ctx->device);
+    free_stack(s, false);
+}
+
+float fg_context_memory_fraction(const fg_context* ctx) { return ctx ? ctx->memory_fraction : 0.0f; }
+
+extern "C" {
+
 static int add_frame(fg_stack* s, const fg_image_desc* d, const void* px, bool dev, const fg_pose* pose,
                      const fg_rect* roi) {
     if (!s) return fail(FG_ERR_INVALID_ARG, "null stack");
@@ -826,12 +868,12 @@ static int add_frame(fg_stack* s, const fg_image_desc* d, const void* px, bool d
     s->added++;
     // uncached stacks keep one frame slot: run pass 1 now; single-pass stacks when their group
     // of slots is full; otherwise when the host staging is full
+    const bool ring_full = !dev && s->stage_used == s->stage_slots;
     if (s->single) {
-        if (s->added - s->p1_done >= (s->slots ? s->slots : kStageSlots) || s->stage_used == kStageSlots)
-            return flush_pass1(s);
+        if (s->added - s->p1_done >= (s->slots ? s->slots : kStageSlots) || ring_full) return flush_pass1(s);
         return FG_OK;
     }
-    if (!s->cached || s->stage_used == kStageSlots) return flush_pass1(s);
+    if (!s->cached || ring_full) return flush_pass1(s);
     return FG_OK;
 }
 

@@ -19,7 +19,11 @@ struct fg_dev_allocator {
 // memory fraction of free HBM) and its buffers from `allocator` (nullptr: hipMalloc).
 int fg_stack_create_budget(fg_context* ctx, const fg_stack_opts* o, double budget, fg_stack** out,
                            const fg_dev_allocator* allocator = nullptr);
-// HIP device of a context.
+// HIP device of a context, and its --memory-fraction.
 int fg_context_device(const fg_context* ctx);
+float fg_context_memory_fraction(const fg_context* ctx);
+// fg_stack_destroy for a stack whose work is known complete: does not wait on its stream (which
+// may be shared with, and busy running, another stack).
+void fg_stack_destroy_idle(fg_stack* stack);
 // Sets the calling thread's fg_last_error() message and returns `code`.
 int fg_set_error(int code, const char* msg);

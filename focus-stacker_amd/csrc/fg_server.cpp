@@ -239,7 +239,7 @@ void fg_server::start(int slot, const std::shared_ptr<Request>& r) {
 void fg_server::drop_idle() {
     for (auto& pool : idle) {
         for (Pooled& p : pool) {
-            fg_stack_destroy(p.stack);
+            fg_stack_destroy_idle(p.stack);
             in_use -= (double)p.bytes;
         }
         pool.clear();
@@ -259,7 +259,7 @@ void fg_server::retire(int slot, int state) {
                 size_t lru = 0;
                 for (size_t i = 1; i < pool.size(); i++)
                     if (pool[i].last_use < pool[lru].last_use) lru = i;
-                fg_stack_destroy(pool[lru].stack);
+                fg_stack_destroy_idle(pool[lru].stack);
                 in_use -= (double)pool[lru].bytes;
                 pool.erase(pool.begin() + (long)lru);
             }
@@ -303,7 +303,7 @@ bool fg_server::admit(std::unique_lock<std::mutex>&) {
                             bi = i;
                         }
                 if (bs < 0) break;
-                fg_stack_destroy(idle[bs][bi].stack);
+                fg_stack_destroy_idle(idle[bs][bi].stack);
                 in_use -= (double)idle[bs][bi].bytes;
                 idle[bs].erase(idle[bs].begin() + (long)bi);
                 need = fg_plan_stack_bytes(&r->opts, budget > in_use ? budget - in_use : 0.0, &cached);
@@ -365,7 +365,9 @@ int fg_server_create(fg_context* ctx, const fg_server_opts* opts, fg_server** ou
     }
     size_t free_b = 0, total_b = 0;
     (void)hipMemGetInfo(&free_b, &total_b);
-    const float frac = opts && opts->memory_fraction > 0.0f ? opts->memory_fraction : 0.0f;
+    // 0: the context's --memory-fraction; <= 0 there too: all free HBM but 1 GB
+    float frac = opts && opts->memory_fraction > 0.0f ? opts->memory_fraction : 0.0f;
+    if (frac <= 0.0f) frac = fg_context_memory_fraction(ctx);
     s->budget = frac > 0.0f ? (double)frac * (double)free_b : (double)free_b - 1.0e9;
     s->arena.limit = s->budget > 0.0 ? (size_t)s->budget : 0;
     s->hook.user = &s->arena;

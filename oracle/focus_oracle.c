@@ -21,9 +21,38 @@
 #include <stdlib.h>
 #include <string.h>
 #include <float.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 
 typedef or_real R;
 static const double PI = 3.14159265358979323846;
+
+/* Threading.  Loops whose iterations are independent (one output pixel or row each) run in
+ * parallel; every iteration performs exactly the operations of the serial loop, so the result is
+ * the same bits for any thread count.  The reference's own order-dependent sums (the contrast's
+ * sliding column sums, the norm-sum and accumulator frame order, the no-wrap expand's running
+ * sums) keep their serial order (or_contrast splits its slide into a per-column phase and a
+ * per-row phase, each in the reference's order). */
+#define OR_PAR_MIN 16384
+#define OR_STR(x) #x
+#define OR_PRAGMA(x) _Pragma(OR_STR(x))
+#define PAR_FOR(n) OR_PRAGMA(omp parallel for schedule(static) if ((size_t)(n) >= OR_PAR_MIN))
+
+void or_set_threads(int n) {
+#ifdef _OPENMP
+    omp_set_num_threads(n > 0 ? n : 1);
+#else
+    (void)n;
+#endif
+}
+int or_get_threads(void) {
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}
 
 /* ------------------------------------------------------------------------- */
 /* pose math — src/optimize.zig                                               */
@@ -452,6 +481,7 @@ static void undistort_source(const or_inv_cache* c, double dx, double dy, double
             double stx = c.world_to_local[0][0], sty = c.world_to_local[1][0], stz = c.world_to_local[2][0]; \
             double cz = -c.dest_focal, bx = rl - c.dest_center_x;                                       \
             int radial = c.radial_a != 0.0 || c.radial_b != 0.0 || c.radial_c != 0.0;                  \
+            PAR_FOR((size_t)ow * oh)                                                                    \
             for (uint32_t y = 0; y < oh; y++) {                                                         \
                 double py = (rt + (double)y) - c.dest_center_y;                                         \
                 double lx = c.world_to_local[0][0] * bx + c.world_to_local[0][1] * py + c.world_to_local[0][2] * cz; \
@@ -475,6 +505,7 @@ static void undistort_source(const or_inv_cache* c, double dx, double dy, double
             }                                                                                           \
             return;                                                                                     \
         }                                                                                               \
+        PAR_FOR((size_t)ow * oh)                                                                        \
         for (uint32_t y = 0; y < oh; y++) {                                                             \
             double wy = rt + (double)y, wx = rl;                                                        \
             T* d = dst + (size_t)y * ow * dch;                                                          \
@@ -500,6 +531,7 @@ static inline uint32_t sample_at(const void* img, int sb, size_t i) {
 /* fillAverageU8/U16 src/fuse/grayscale.zig:21-57 */
 void or_gray(const void* img, int sb, uint32_t w, uint32_t h, int color, int extra, R* gray) {
     size_t n = (size_t)w * h, st = (size_t)(color + extra);
+    PAR_FOR(n)
     for (size_t i = 0; i < n; i++) {
         if (color == 1) {
             gray[i] = sb == 1 ? (R)sample_at(img, sb, i * st) / (R)255.0 : (R)sample_at(img, sb, i * st) / (R)65535.0;
@@ -512,11 +544,13 @@ void or_gray(const void* img, int sb, uint32_t w, uint32_t h, int color, int ext
 /* fillBinarySupport src/fuse/masks.zig:32-57 */
 void or_binary_support(const void* img, int sb, uint32_t w, uint32_t h, int color, int extra, R* out) {
     size_t n = (size_t)w * h, st = (size_t)(color + extra);
+    PAR_FOR(n)
     for (size_t i = 0; i < n; i++) out[i] = extra == 0 ? (R)1 : (sample_at(img, sb, i * st + color) > 0 ? (R)1 : (R)0);
 }
 /* accumulateBinarySupportMax src/fuse/masks.zig:59-84 */
 void or_union_support(const void* img, int sb, uint32_t w, uint32_t h, int color, int extra, R* out) {
     size_t n = (size_t)w * h, st = (size_t)(color + extra);
+    PAR_FOR(n)
     for (size_t i = 0; i < n; i++)
         if (extra == 0 || sample_at(img, sb, i * st + color) > 0) out[i] = (R)1;
 }
@@ -524,6 +558,7 @@ void or_union_support(const void* img, int sb, uint32_t w, uint32_t h, int color
 void or_apply_support(const void* img, int sb, uint32_t w, uint32_t h, int color, int extra, R* weights) {
     if (extra == 0) return;
     size_t n = (size_t)w * h, st = (size_t)(color + extra);
+    PAR_FOR(n)
     for (size_t i = 0; i < n; i++) {
         R s = (R)sample_at(img, sb, i * st + color) / (sb == 1 ? (R)255.0 : (R)65535.0);
         weights[i] *= s;
@@ -549,48 +584,59 @@ void or_contrast(const R* px, const R* support, uint32_t w, uint32_t h, uint32_t
     uint32_t border = window / 2;
     uint32_t first_row = border, last_row = h - border;
     if (first_row >= last_row) return;
-    double* sums = calloc(w, sizeof(double));
-    double* sqr = calloc(w, sizeof(double));
-    uint32_t* cnt = calloc(w, sizeof(uint32_t));
+    /* Phase 1, per column (independent columns): the column sums of each output row, formed by
+     * the reference's slide (initial window top to bottom, then per row: remove the row leaving,
+     * add the row entering).  Phase 2, per row (independent rows): the horizontal slide over those
+     * sums.  Every value takes the operations of the single-threaded slide in its order. */
+    size_t rows = last_row - first_row;
+    double* sums = malloc(sizeof(double) * rows * w);
+    double* sqr = malloc(sizeof(double) * rows * w);
+    uint32_t* cnt = malloc(sizeof(uint32_t) * rows * w);
+    PAR_FOR((size_t)w * h)
     for (uint32_t c = 0; c < w; c++) {
         double cs = 0, cq = 0;
-        cnt[c] = 0;
+        uint32_t cn = 0;
         for (uint32_t yy = first_row - border; yy <= first_row + border; yy++) {
             size_t idx = (size_t)yy * w + c;
             if (support[idx] <= (R)0) continue;
             R v = px[idx];
             cs += (double)v;
             cq += (double)(R)(v * v);
-            cnt[c] += 1;
+            cn += 1;
         }
-        sums[c] = cs; sqr[c] = cq;
+        for (uint32_t row = first_row; row < last_row; row++) {
+            size_t o = (size_t)(row - first_row) * w + c;
+            sums[o] = cs; sqr[o] = cq; cnt[o] = cn;
+            if (row + 1 >= last_row) break;
+            size_t rb = (size_t)(row - border) * w + c, ab = (size_t)(row + border + 1) * w + c;
+            if (support[rb] > (R)0) {
+                R v = px[rb];
+                cs -= (double)v; cq -= (double)(R)(v * v); cn -= 1;
+            }
+            if (support[ab] > (R)0) {
+                R v = px[ab];
+                cs += (double)v; cq += (double)(R)(v * v); cn += 1;
+            }
+        }
     }
+    PAR_FOR((size_t)w * h)
     for (uint32_t row = first_row; row < last_row; row++) {
+        const double* rs = sums + (size_t)(row - first_row) * w;
+        const double* rq = sqr + (size_t)(row - first_row) * w;
+        const uint32_t* rn = cnt + (size_t)(row - first_row) * w;
         double sum = 0, sum_sqr = 0;
         uint32_t n = 0;
-        for (uint32_t c = 0; c < window; c++) { sum += sums[c]; sum_sqr += sqr[c]; n += cnt[c]; }
+        for (uint32_t c = 0; c < window; c++) { sum += rs[c]; sum_sqr += rq[c]; n += rn[c]; }
         uint32_t x = border;
         size_t out = (size_t)row * w + border;
         for (;;) {
             weights[out] = support[out] > (R)0 ? (R)or_sample_std(sum, sum_sqr, n) : (R)0;
             if (x == w - border - 1) break;
             uint32_t nx = x + border + 1, old = x - border;
-            sum += sums[nx] - sums[old];
-            sum_sqr += sqr[nx] - sqr[old];
-            n += cnt[nx] - cnt[old];
+            sum += rs[nx] - rs[old];
+            sum_sqr += rq[nx] - rq[old];
+            n += rn[nx] - rn[old];
             x++; out++;
-        }
-        if (row + 1 >= last_row) continue;
-        size_t rb = (size_t)(row - border) * w, ab = (size_t)(row + border + 1) * w;
-        for (uint32_t c = 0; c < w; c++) {
-            if (support[rb + c] > (R)0) {
-                R v = px[rb + c];
-                sums[c] -= (double)v; sqr[c] -= (double)(R)(v * v); cnt[c] -= 1;
-            }
-            if (support[ab + c] > (R)0) {
-                R v = px[ab + c];
-                sums[c] += (double)v; sqr[c] += (double)(R)(v * v); cnt[c] += 1;
-            }
         }
     }
     free(sums); free(sqr); free(cnt);
@@ -599,6 +645,7 @@ void or_contrast(const R* px, const R* support, uint32_t w, uint32_t h, uint32_t
 /* normalizeWeightsInto src/fuse/pyramid.zig:303-316 */
 void or_normalize(const R* wgt, const R* norm, uint32_t count, uint32_t total, R* out) {
     R def = (R)1.0 / (R)total;
+    PAR_FOR(count)
     for (uint32_t i = 0; i < count; i++) out[i] = norm[i] > (R)1e-12 ? wgt[i] / norm[i] : def;
 }
 
@@ -615,6 +662,7 @@ static const R KW[5] = {1, 4, 6, 4, 1}; /* kernelWeight src/fuse/pyramid.zig:148
 /* reduceScalarWithSupportRowsSequential src/fuse/pyramid.zig:997-1042 */
 void or_reduce_scalar_support(uint32_t sw, uint32_t sh, const R* src, const R* sup, uint32_t dw, uint32_t dh,
                               R* dst, R* dsup) {
+    PAR_FOR((size_t)dw * dh)
     for (uint32_t dy = 0; dy < dh; dy++)
         for (uint32_t dx = 0; dx < dw; dx++) {
             int32_t cy = (int32_t)dy * 2, cx = (int32_t)dx * 2;
@@ -643,6 +691,7 @@ void or_reduce_scalar_support(uint32_t sw, uint32_t sh, const R* src, const R* s
 /* reduceRgbWithSupportRowsSequential src/fuse/pyramid.zig:1044-1099 */
 void or_reduce_rgb_support(uint32_t sw, uint32_t sh, const R* src, const R* sup, uint32_t dw, uint32_t dh,
                            R* dst, R* dsup) {
+    PAR_FOR((size_t)dw * dh)
     for (uint32_t dy = 0; dy < dh; dy++)
         for (uint32_t dx = 0; dx < dw; dx++) {
             int32_t cy = (int32_t)dy * 2, cx = (int32_t)dx * 2;
@@ -682,7 +731,10 @@ static uint32_t clamp_coord(int32_t c, uint32_t limit) {
 
 /* reduceRgbRowsSequential src/fuse/pyramid.zig:952-995 with horizontalReduceRgbRow :1726-1746 */
 void or_reduce_rgb_clamped(uint32_t sw, uint32_t sh, const R* src, uint32_t dw, uint32_t dh, R* dst) {
+    OR_PRAGMA(omp parallel if ((size_t)dw * dh >= OR_PAR_MIN))
+    {
     R* rows = malloc(sizeof(R) * 5 * (size_t)dw * 3);
+    OR_PRAGMA(omp for schedule(static))
     for (uint32_t dy = 0; dy < dh; dy++) {
         for (int ky = 0; ky < 5; ky++) {
             uint32_t y = clamp_coord((int32_t)dy * 2 + ky - 2, sh);
@@ -706,6 +758,7 @@ void or_reduce_rgb_clamped(uint32_t sw, uint32_t sh, const R* src, uint32_t dw, 
             }
     }
     free(rows);
+    }
 }
 
 /* horizontalExpandRgbRow src/fuse/pyramid.zig:1748-1775 */
@@ -727,9 +780,12 @@ static void h_expand_row(uint32_t sw, const R* srow, uint32_t dw, R* out) {
  * reference fetches y0 = y1 = clamp(dy/2) (:1127-1128), so the row is the horizontal
  * expansion of source row dy/2 alone, not the average of rows dy/2 and dy/2+1. */
 static void expand_clamp(uint32_t dw, uint32_t dh, uint32_t sw, uint32_t sh, const R* src, R* dst) {
+    OR_PRAGMA(omp parallel if ((size_t)dw * dh >= OR_PAR_MIN))
+    {
     R* r0 = malloc(sizeof(R) * (size_t)dw * 3);
     R* r1 = malloc(sizeof(R) * (size_t)dw * 3);
     R* r2 = malloc(sizeof(R) * (size_t)dw * 3);
+    OR_PRAGMA(omp for schedule(static))
     for (uint32_t dy = 0; dy < dh; dy++) {
         int even = (dy & 1) == 0;
         int32_t by = (int32_t)(dy / 2);
@@ -744,6 +800,7 @@ static void expand_clamp(uint32_t dw, uint32_t dh, uint32_t sw, uint32_t sh, con
             d[k] = even ? (r0[k] + (R)6.0 * r1[k] + r2[k]) * ((R)1.0 / (R)8.0) : (r0[k] + r1[k]) * (R)0.5;
     }
     free(r0); free(r1); free(r2);
+    }
 }
 
 /* expandRgbExactNoWrap src/fuse/pyramid.zig:1167-1354, restated with the same running
@@ -891,6 +948,7 @@ void or_expand_rgb(uint32_t dw, uint32_t dh, uint32_t sw, uint32_t sh, const R* 
 /* fillRgbBaseRows src/fuse/pyramid.zig:606-639 */
 void or_fill_rgb_base(const void* img, int sb, uint32_t w, uint32_t h, int color, int extra, R* dst) {
     size_t n = (size_t)w * h, st = (size_t)(color + extra);
+    PAR_FOR(n)
     for (size_t i = 0; i < n; i++)
         for (int c = 0; c < 3; c++) dst[i * 3 + c] = (R)sample_at(img, sb, i * st + c);
 }
@@ -906,6 +964,7 @@ static void level_dims(uint32_t w, uint32_t h, uint32_t levels, uint32_t* lw, ui
     off[levels] = o;
 }
 static void quantize(const R* base, size_t count, int sb, void* out) {
+    PAR_FOR(count)
     for (size_t i = 0; i < count; i++) {
         R v = base[i] + (R)0.5;
         R hi = sb == 1 ? (R)255.0 : (R)65535.0;
@@ -924,7 +983,9 @@ void or_collapse(uint32_t w, uint32_t h, uint32_t levels, const R* acc, int sb, 
     for (uint32_t l = levels - 1; l >= 1; l--) {
         R* parent = c + off[l - 1] * 3;
         or_expand_rgb(lw[l - 1], lh[l - 1], lw[l], lh[l], c + off[l] * 3, tmp);
-        for (size_t k = 0; k < (size_t)lw[l - 1] * lh[l - 1] * 3; k++) parent[k] += tmp[k];
+        size_t pn = (size_t)lw[l - 1] * lh[l - 1] * 3;
+        PAR_FOR(pn)
+        for (size_t k = 0; k < pn; k++) parent[k] += tmp[k];
     }
     quantize(c, (size_t)w * h * 3, sb, out);
     free(c); free(tmp);
@@ -949,16 +1010,23 @@ void or_merge_hybrid(const void* soft, const void* hard, int sb, uint32_t w, uin
             or_reduce_rgb_clamped(lw[l - 1], lh[l - 1], g + off[l - 1] * 3, lw[l], lh[l], g + off[l] * 3);
         for (uint32_t l = 0; l + 1 < levels; l++) {
             or_expand_rgb(lw[l], lh[l], lw[l + 1], lh[l + 1], g + off[l + 1] * 3, tmp);
-            for (size_t i = 0; i < (size_t)lw[l] * lh[l] * 3; i++) lap[k][off[l] * 3 + i] = g[off[l] * 3 + i] - tmp[i];
+            size_t ln = (size_t)lw[l] * lh[l] * 3;
+            R* lk = lap[k];
+            PAR_FOR(ln)
+            for (size_t i = 0; i < ln; i++) lk[off[l] * 3 + i] = g[off[l] * 3 + i] - tmp[i];
         }
         memcpy(lap[k] + off[levels - 1] * 3, g + off[levels - 1] * 3, sizeof(R) * (size_t)lw[levels - 1] * lh[levels - 1] * 3);
     }
     uint32_t hs = start < levels ? start : levels;
     uint32_t he = hs + count < levels ? hs + count : levels;
     R m = (R)mix;
-    for (uint32_t l = 0; l < levels; l++)
+    for (uint32_t l = 0; l < levels; l++) {
+        R* l0 = lap[0];
+        const R* l1 = lap[1];
+        PAR_FOR((off[l + 1] - off[l]) * 3)
         for (size_t i = off[l] * 3; i < off[l + 1] * 3; i++)
-            lap[0][i] = (l >= hs && l < he) ? lap[0][i] + (lap[1][i] - lap[0][i]) * m : lap[0][i];
+            l0[i] = (l >= hs && l < he) ? l0[i] + (l1[i] - l0[i]) * m : l0[i];
+    }
     or_collapse(w, h, levels, lap[0], sb, out);
     free(lap[0]); free(lap[1]); free(g); free(tmp);
 }
@@ -968,6 +1036,7 @@ void or_merge_hybrid(const void* soft, const void* hard, int sb, uint32_t w, uin
 void or_update_winners(const void* img, int sb, uint32_t w, uint32_t h, int color, int extra, const R* raw,
                        R* best, void* out) {
     size_t n = (size_t)w * h, st = (size_t)(color + extra);
+    PAR_FOR(n)
     for (size_t i = 0; i < n; i++) {
         R wt = raw[i];
         if (extra) wt = wt * ((R)sample_at(img, sb, i * st + color) / (sb == 1 ? (R)255.0 : (R)65535.0));
@@ -984,6 +1053,7 @@ void or_update_winners(const void* img, int sb, uint32_t w, uint32_t h, int colo
 /* blurFiveTapInto src/fuse/masks.zig:86-105 with horizontalPassRange :151-177 and
  * verticalPassRange :179-207: clamped (1,4,6,4,1)/16, summed left to right, rows then columns */
 void or_blur_five_tap(uint32_t w, uint32_t h, const R* in, R* scratch, R* out) {
+    PAR_FOR((size_t)w * h)
     for (size_t y = 0; y < h; y++) {
         const R* r = in + y * w;
         for (size_t x = 0; x < w; x++) {
@@ -992,6 +1062,7 @@ void or_blur_five_tap(uint32_t w, uint32_t h, const R* in, R* scratch, R* out) {
             scratch[y * w + x] = ((((r[xm2] + (R)4.0 * r[xm1]) + (R)6.0 * r[x]) + (R)4.0 * r[xp1]) + r[xp2]) / (R)16.0;
         }
     }
+    PAR_FOR((size_t)w * h)
     for (size_t y = 0; y < h; y++) {
         size_t ym2 = y >= 2 ? y - 2 : 0, ym1 = y >= 1 ? y - 1 : 0;
         size_t yp1 = y + 1 < h ? y + 1 : h - 1, yp2 = y + 2 < h ? y + 2 : h - 1;
@@ -1007,6 +1078,7 @@ void or_blur_five_tap(uint32_t w, uint32_t h, const R* in, R* scratch, R* out) {
 void or_soft_accumulate(const void* img, int sb, uint32_t w, uint32_t h, int color, int extra, const R* wts,
                         R* ws, R* ps) {
     size_t n = (size_t)w * h, st = (size_t)(color + extra);
+    PAR_FOR(n)
     for (size_t i = 0; i < n; i++) {
         R wt = wts[i];
         if (wt <= 0) continue;
@@ -1019,6 +1091,7 @@ void or_soft_accumulate(const void* img, int sb, uint32_t w, uint32_t h, int col
 void or_soft_finalize(const R* ws, const R* ps, uint32_t w, uint32_t h, int sb, void* out) {
     size_t n = (size_t)w * h;
     R hi = sb == 1 ? (R)255.0 : (R)65535.0;
+    PAR_FOR(n)
     for (size_t i = 0; i < n; i++) {
         R wt = ws[i];
         for (int c = 0; c < 3; c++) {
@@ -1070,6 +1143,7 @@ static void accumulate_frame(const void* img, int sb, uint32_t w, uint32_t h, in
         const R* cur = glev + off[l] * 3;
         const R* m = mlev + off[l];
         R* d = acc + off[l] * 3;
+        PAR_FOR((size_t)lw[l] * lh[l])
         for (size_t p = 0; p < (size_t)lw[l] * lh[l]; p++) {
             R wt = m[p];
             R l0 = cur[p * 3] - tmp[p * 3], l1 = cur[p * 3 + 1] - tmp[p * 3 + 1], l2 = cur[p * 3 + 2] - tmp[p * 3 + 2];
@@ -1144,6 +1218,7 @@ int or_fuse_frames(int n, const void* const* frames, uint32_t w, uint32_t h, int
         weight_map(frames[k], sb, w, h, extra, o->window, gray, sup, wk);
         or_apply_support(frames[k], sb, w, h, 3, extra, wk);
         or_union_support(frames[k], sb, w, h, 3, extra, U);
+        PAR_FOR(count)
         for (size_t i = 0; i < count; i++) S[i] += wk[i];
     }
     /* pass 2: src/stack/pipeline.zig:526-597 */

@@ -50,6 +50,11 @@ typedef struct {
     double plane_normal[3];
 } or_inv_cache;
 
+/* Threads of the parallel loops (OpenMP; default: OMP_NUM_THREADS or the cores available).
+ * The results are the same bits for every thread count. */
+void or_set_threads(int n);
+int or_get_threads(void);
+
 /* ---- pose math (a1-a3, a6) ---- */
 int or_pose_is_basic_rectilinear(const or_pose* pose);
 int or_pose_has_translation(const or_pose* pose);

@@ -130,6 +130,15 @@ class Oracle:
         L.or_soft_finalize.argtypes = [vp, vp, u32, u32, i32, vp]
         L.or_merge_hybrid.argtypes = [vp, vp, i32, u32, u32, u32, u32, C.c_float, u32, vp]
         L.or_collapse.argtypes = [u32, u32, u32, vp, i32, vp]
+        L.or_set_threads.argtypes = [i32]
+        L.or_get_threads.restype = i32
+
+    # ---- threads of the parallel loops (the same bits at any count) ----
+    def set_threads(self, n: int) -> None:
+        self.lib.or_set_threads(int(n))
+
+    def threads(self) -> int:
+        return int(self.lib.or_get_threads())
 
     # ---- pose math ----
     def inverse_cache(self, pose: Pose, w: int, h: int) -> InvCache:

@@ -348,8 +348,9 @@ static void gpu_checks(const char* dir) {
             CHECK(or_stack(NF, (const void* const*)frames, SW, SH, bytes, (const or_pose*)poses, oroi, &oo, ref, NULL) == 0,
                   "oracle");
             if (m == FG_METHOD_HARDMASK_CONTRAST) {
+                /* winners: a strict > over weights both sides round identically (8-bit frames) */
                 const double frac = pixels_differing(out, ref, npx, bytes);
-                CHECK(frac < 1e-3, "hardmask vs oracle: %.2e of the pixels differ", frac);
+                CHECK(frac == 0.0, "hardmask vs oracle: %.2e of the pixels differ", frac);
             } else {
                 CHECK(images_close(out, ref, npx, bytes, &maxd, &meand), "method %u vs oracle: max %d mean %.4f", m,
                       maxd, meand);
@@ -457,8 +458,7 @@ static void gpu_checks(const char* dir) {
                 differ += d != 0;
                 if (d > maxd) maxd = d;
             }
-            CHECK(maxd <= 1 && differ * 1000 < (size_t)W * H * 4, "remap vs oracle: max %d, %zu samples differ", maxd,
-                  differ);
+            CHECK(differ == 0, "remap vs oracle: max %d, %zu samples differ", maxd, differ);
             snprintf(prefix, sizeof prefix, "%s/fg_conformance_%d", dir, bytes);
             CHECK(fg_write_aligned_tiff(ctx, &sd, frames[2], &poses[2], &roi, prefix, 2) == FG_OK, "write_aligned_tiff");
             snprintf(path, sizeof path, "%s_0002.tif", prefix);

@@ -1,13 +1,21 @@
 """GPU: whole stacks at the real frame size, 7952x5304, through the C ABI.
 
-The oracle takes minutes per frame at this size, so these tests check properties that fix the
-output whatever the size:
+Against the oracle (its loops run on every host thread the job has, with the same bits as one
+thread): short stacks from the seeded generators of the 30-frame u8 and the 143-frame u16
+workloads (the same poses: shifts of +-20 px, roll +-0.3 deg, scale +-1 %; defocus up to 6 px;
+the u16 one with its three focal planes), at full pyramid depth and capped at 9 levels, pyramid
+and hybrid (sharpness 0.35), with the intermediate sums read back.  The frames are generated on
+the GPU and copied to the host for the oracle.
+
+Properties that fix the output whatever the size:
 - N identical frames reproduce the input. Every mask level is then 1/N, and the collapse of a
   Laplacian pyramid returns its base (src/fuse/pyramid.zig:223-269, 303-316).
 - The hybrid at sharpness 0 returns the pyramid image. The merge then keeps the soft Laplacian
   (mergeHybridImages src/fuse/pyramid.zig:271-301), and the soft image is the pyramid result.
 Both hold to well under half a sample step, so the quantized outputs must match exactly.
 """
+import os
+
 import numpy as np
 import pytest
 
@@ -82,3 +90,133 @@ def test_hybrid_at_sharpness_zero_is_the_pyramid(ctx, fg, dtype):
         outs[name] = s.finish()
         s.close()
     assert np.array_equal(outs["hybrid"], outs["pyramid"])
+
+
+# --------------------------------------------------------------------------
+# against the oracle
+# --------------------------------------------------------------------------
+
+def _img_close(a, b, max_abs=1, mean_abs=0.05):
+    d = np.abs(a.astype(np.int64) - b.astype(np.int64))
+    assert d.max() <= max_abs, f"max |d| = {d.max()}"
+    assert d.mean() < mean_abs, f"mean |d| = {d.mean()}"
+    return d
+
+
+def _level_close(got, ref, rel=1e-5):
+    rms = float(np.sqrt(np.mean(np.square(ref.astype(np.float64)))))
+    err = np.abs(got.astype(np.float64) - ref.astype(np.float64))
+    bad = err > rel * np.maximum(np.abs(ref.astype(np.float64)), rms) + 1e-30
+    assert not bad.any(), f"{bad.sum()} of {bad.size} values off; worst {err.max()} (rms {rms})"
+
+
+def _generated(config_id, frames):
+    """Frames 0..frames-1 of a stack of `frames` from the workload's seeded generator, made on the
+    GPU, as host arrays; the poses; the common-overlap ROI."""
+    import torch
+    import synth
+    import focus_gpu
+    spec = synth.config_spec(config_id, frames=frames)
+    st = synth.Stack(spec, device=torch.device("cuda:0"))
+    out = [st.frame_numpy(k) for k in range(frames)]
+    poses = st.poses
+    del st
+    torch.cuda.empty_cache()
+    roi = focus_gpu.common_overlap_roi([(spec.width, spec.height)] * frames, [focus_gpu.pose(p) for p in poses])
+    return out, poses, roi
+
+
+@pytest.fixture(scope="module")
+def gen_u8():
+    print("generating 4 frames of the 7952x5304 u8 workload on the GPU", flush=True)
+    return _generated(1, 4)
+
+
+@pytest.fixture(scope="module")
+def gen_u16():
+    print("generating 3 frames of the 7952x5304 u16 three-plane workload on the GPU", flush=True)
+    return _generated(2, 3)
+
+
+def _oracle():
+    from oracle import Oracle
+    o = Oracle()
+    n = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or len(os.sched_getaffinity(0))
+    o.set_threads(n)
+    return o
+
+
+def _gpu_stack(fg, ctx, frames, poses, roi, **kw):
+    w, h = roi[2] - roi[0], roi[3] - roi[1]
+    st = fg.SAMPLE_U8 if frames[0].dtype == np.uint8 else fg.SAMPLE_U16
+    s = fg.Stack(ctx, len(frames), w, h, st, **kw)
+    for f, p in zip(frames, poses):
+        s.add_frame(f, fg.pose(p), roi)
+    assert not s.needs_pass2_frames()
+    return s, s.finish()
+
+
+@pytest.mark.parametrize("levels", [0, 9])
+def test_u8_workload_frames_match_oracle(ctx, fg, gen_u8, levels):
+    """Four frames of the 30 x 7952x5304 u8 workload's generator, remap + pyramid-contrast at the
+    reference's depth (14 levels) and at the 9 levels the workload names."""
+    from oracle import make_pose
+    frames, poses, roi = gen_u8
+    print(f"u8 workload, 4 frames, roi={roi}, levels={levels}", flush=True)
+    s, out = _gpu_stack(fg, ctx, frames, poses, roi, max_levels=levels)
+    assert len(s.levels()) == (14 if levels == 0 else 9)
+    o = _oracle()
+    ref, dbg = o.stack(frames, [make_pose(**p) for p in poses], roi, max_levels=levels, debug=True)
+    d = _img_close(out, ref)
+    print(f"  max {d.max()} mean {d.mean():.2e} differing {float((d != 0).mean()):.2e}", flush=True)
+    assert np.array_equal(s.readback("union_support"), dbg["union_support"])
+    _level_close(s.readback("norm_sum"), dbg["norm_sum"])
+    for k in (0, 3):
+        _level_close(s.readback("weights", k), dbg["weights"][k])
+    s.close()
+
+
+@pytest.mark.parametrize("method", ["pyramid", "hybrid"])
+def test_u16_workload_frames_match_oracle(ctx, fg, gen_u16, method):
+    """Three frames of the 143 x 7952x5304 u16 three-plane workload's generator: pyramid-contrast,
+    and hybrid-pyramid-contrast at sharpness 0.35 (its soft and hard images checked too)."""
+    from oracle import make_pose, METHOD_PYRAMID, METHOD_HYBRID
+    frames, poses, roi = gen_u16
+    print(f"u16 workload, 3 frames, roi={roi}, {method}", flush=True)
+    m = fg.METHOD_PYRAMID if method == "pyramid" else fg.METHOD_HYBRID
+    s, out = _gpu_stack(fg, ctx, frames, poses, roi, method=m, sharpness=0.35)
+    o = _oracle()
+    om = METHOD_PYRAMID if method == "pyramid" else METHOD_HYBRID
+    ref, dbg = o.stack(frames, [make_pose(**p) for p in poses], roi, method=om, sharpness=0.35, debug=True)
+    d = _img_close(out, ref)
+    print(f"  max {d.max()} mean {d.mean():.2e} differing {float((d != 0).mean()):.2e}", flush=True)
+    _level_close(s.readback("norm_sum"), dbg["norm_sum"])
+    if method == "hybrid":
+        _img_close(s.readback("soft"), dbg["soft"])
+        hard = s.readback("hard")
+        differ = (hard != dbg["hard"]).any(axis=2)
+        top2 = np.sort(dbg["weights"].astype(np.float64), axis=0)[-2:]
+        tied = (top2[1] - top2[0]) <= 1e-6 * np.maximum(top2[1], 1e-30)
+        print(f"  hard image: {int(differ.sum())} winners differ, {int((differ & ~tied).sum())} away from a tie",
+              flush=True)
+        assert not (differ & ~tied).any()
+    s.close()
+
+
+def test_u8_workload_hardmask_softmask_match_oracle(ctx, fg, gen_u8):
+    """The single-pass methods on the same full-size frames: winners bit-exact, softmask within
+    the image tolerance."""
+    from oracle import make_pose, METHOD_HARDMASK, METHOD_SOFTMASK
+    frames, poses, roi = gen_u8
+    o = _oracle()
+    opo = [make_pose(**p) for p in poses]
+    for name, m, om in (("hardmask", fg.METHOD_HARDMASK, METHOD_HARDMASK),
+                        ("softmask", fg.METHOD_SOFTMASK, METHOD_SOFTMASK)):
+        print(f"u8 workload, 4 frames, {name}", flush=True)
+        s, out = _gpu_stack(fg, ctx, frames, poses, roi, method=m)
+        s.close()
+        ref = o.stack(frames, opo, roi, method=om)
+        if name == "hardmask":
+            assert np.array_equal(out, ref), f"{int((out != ref).any(axis=2).sum())} winners differ"
+        else:
+            _img_close(out, ref)

@@ -3,9 +3,13 @@
 Tolerances (stated by the task for this path):
   * 8/16-bit final images: max |d| <= 1 LSB and mean |d| < 0.05 LSB;
   * float pyramid / accumulator levels: |d| <= 1e-5 * max(|ref|, RMS of the level);
-  * remap output: integer samples, checked the same way as final images and, in addition,
-    reported as the fraction of samples that differ (expected to be ~0: the GPU evaluates
-    the same f64 formulas, the reference's legacy trig chain as its equivalent homography).
+  * remap output and hard-mask winners: integer and index results, asserted bit-exact.  The GPU
+    evaluates the reference's f64 remap formulas (the legacy trig chain as its equivalent
+    homography) and picks winners with the same strict > over weights that round identically.
+    The one exception: at 16 bits the contrast's f64 sum of squares is not exact, and the GPU adds
+    the window in another order than the reference's sliding sum, so a weight may differ in its
+    last bit; a winner may then differ only at a pixel where two frames' weights tie to within
+    that rounding (_winners_exact_or_tied).
 """
 import json
 import os
@@ -52,16 +56,33 @@ def _level_close(got, ref, rel=1e-5):
 # --------------------------------------------------------------------------
 
 def _remap_close(got, ref):
-    """Alpha always compared; colour compared where either side has support.  A source
-    coordinate that lands exactly on the -1 boundary row/column (identity-like poses with
-    the ROI outside the frame) comes out of the reference's trig chain as -1 +- 1 ulp, so
-    there the colour of a zero-support sample is not defined by the reference either."""
+    """Bit-exact, alpha included; the colour of a sample is compared where either side has
+    support.  A source coordinate that lands exactly on the -1 boundary row/column (identity-like
+    poses with the ROI outside the frame) comes out of the reference's trig chain as -1 +- 1 ulp,
+    so there the colour of a zero-support sample is not defined by the reference either."""
     unsupported = (got[..., -1] == 0) & (ref[..., -1] == 0)
     g = got.astype(np.int64)
     r = ref.astype(np.int64)
     g[unsupported, :-1] = 0
     r[unsupported, :-1] = 0
-    return _img_close(g, r, 1, 0.01)
+    d = np.abs(g - r)
+    assert not d.any(), f"{int((d != 0).sum())} samples differ, max {d.max()}"
+    return d
+
+
+def _winners_exact_or_tied(out, ref, weights, sample_bytes):
+    """Hard-mask winners (updateRange src/fuse/blend.zig:237-288: the first frame with a strictly
+    greater weight).  8-bit: every weight is exact in both, so the winners are identical.  16-bit:
+    a pixel may differ only where the oracle's two largest weights are within 1e-6 of each other
+    (the last-bit rounding of the f64 sum of squares)."""
+    differ = (out != ref).any(axis=2)
+    if sample_bytes == 1 or not differ.any():
+        assert not differ.any(), f"{int(differ.sum())} winners differ"
+        return
+    top2 = np.sort(np.asarray(weights, np.float64), axis=0)[-2:]
+    tied = (top2[1] - top2[0]) <= 1e-6 * np.maximum(top2[1], 1e-30)
+    assert not (differ & ~tied).any(), f"{int((differ & ~tied).sum())} winners differ away from a weight tie"
+    print(f"winners: {int(differ.sum())} differ, all at weight ties")
 
 def test_golden_identity_remap(ctx, fg):
     """port3_0000.tif is the reference's own remap output of 0001.jpg under the identity pose."""
@@ -126,10 +147,7 @@ def test_remap_matches_oracle(ctx, fg, oracle, dtype, channels, kind):
     got = ctx.remap(src, fg.pose(POSES[kind]), roi)
     ref = oracle.remap(src, make_pose(**POSES[kind]), roi)
     assert got.shape == ref.shape
-    d = _remap_close(got, ref)
-    frac = float((d != 0).mean())
-    print(f"remap {kind} ch={channels} {np.dtype(dtype).name}: differing samples {frac:.2e}")
-    assert frac < 1e-3
+    _remap_close(got, ref)
 
 
 def test_remap_real_solved_pose(ctx, fg, oracle):
@@ -139,8 +157,7 @@ def test_remap_real_solved_pose(ctx, fg, oracle):
     p = meta["poses"][1]
     got = ctx.remap(d["src"], fg.pose(p))
     ref = oracle.remap(d["src"], make_pose(**p))
-    dd = _remap_close(got, ref)
-    assert (dd != 0).mean() < 1e-4
+    _remap_close(got, ref)
 
 
 # --------------------------------------------------------------------------
@@ -229,8 +246,7 @@ def test_hybrid_matches_oracle(ctx, fg, oracle, sharpness):
     s, out = _run_stack(fg, ctx, frames, poses, roi, method=fg.METHOD_HYBRID, sharpness=sharpness)
     ref, dbg = oracle.stack(frames, opo, roi, method=1, sharpness=sharpness, debug=True)
     _img_close(s.readback("soft"), dbg["soft"])
-    hard = s.readback("hard")
-    assert (hard != dbg["hard"]).mean() < 1e-3   # winners flip only on exact weight ties after ulp noise
+    assert np.array_equal(s.readback("hard"), dbg["hard"])
     _img_close(out, ref)
     s.close()
 
@@ -442,7 +458,7 @@ def test_hybrid_host_frames_several_launches(ctx, fg, oracle):
     s, out = _run_stack(fg, ctx, frames, poses, roi, method=fg.METHOD_HYBRID, debug_frame=1)
     ref, dbg = oracle.stack(frames, opo, roi, method=METHOD_HYBRID, debug=True, frame_index=1)
     _img_close(out, ref)
-    assert (s.readback("hard") != dbg["hard"]).mean() < 1e-3
+    assert np.array_equal(s.readback("hard"), dbg["hard"])
     s.close()
 
 
@@ -577,8 +593,11 @@ def test_random_stacks_match_oracle(ctx, fg, oracle, seed):
     ref = oracle.stack(frames, opo, roi, method=o_m, window=c["window"])
     assert out.shape == ref.shape
     if c["method"] == "hardmask":
-        differ = int((out != ref).any(axis=2).sum())
-        assert differ <= max(1, int(1e-3 * out.shape[0] * out.shape[1])), f"{differ} winners differ"
+        weights = None
+        if c["dtype"] == "u16":
+            _, dbg = oracle.stack(frames, opo, roi, method=om.METHOD_PYRAMID, window=c["window"], debug=True)
+            weights = dbg["weights"]
+        _winners_exact_or_tied(out, ref, weights, 2 if c["dtype"] == "u16" else 1)
     else:
         _img_close(out, ref)
 
@@ -596,8 +615,11 @@ def test_single_pass_methods_match_oracle(ctx, fg, oracle, case):
     assert not s.needs_pass2_frames()
     ref = oracle.stack(frames, opo, roi, method=om)
     if method == "hardmask":
-        # winners flip only where two frames' weights tie to the last ulp
-        assert (out != ref).any(axis=2).mean() < 1e-3
+        weights = None
+        if frames[0].dtype == np.uint16:
+            from oracle import METHOD_PYRAMID
+            weights = oracle.stack(frames, opo, roi, method=METHOD_PYRAMID, debug=True)[1]["weights"]
+        _winners_exact_or_tied(out, ref, weights, frames[0].dtype.itemsize)
     else:
         _img_close(out, ref)
     s.close()

@@ -533,3 +533,161 @@ def test_softmask_matches_numpy_restatement(oracle64):
         ps[pos] += wk[pos][:, None] * f[..., :3][pos]
     ref = np.where(ws[..., None] > 1e-12, np.clip(np.floor(ps / np.where(ws > 0, ws, 1)[..., None] + 0.5), 0, 255), 0)
     assert np.abs(out.astype(int) - ref.astype(int)).max() <= 1
+
+
+# ---- hybrid mix and masked laplacian blend, restated from the pinned primitives ----
+# reduce_clamped, expand (both regimes), reduce_support, contrast, gray, normalise and the
+# collapse's quantisation are each pinned above by brute force or the reference's known answers;
+# these compose them in numpy, as the reference composes them, and check the oracle's own
+# composition against that.  Each test also shows that the mistake it guards against changes the
+# result (so the comparison is not vacuous).
+
+def _lap_pyramid(o, img, levels):
+    g = [np.asarray(img, np.float64)]
+    for _ in range(1, levels):
+        g.append(o.reduce_clamped(g[-1]))
+    lap = [g[l] - o.expand(g[l + 1], g[l].shape[1], g[l].shape[0]) for l in range(levels - 1)]
+    return lap + [g[-1]]
+
+
+def _collapse_quantize(o, levels, hi):
+    c = [l.copy() for l in levels]
+    for l in range(len(c) - 1, 0, -1):
+        c[l - 1] = c[l - 1] + o.expand(c[l], c[l - 1].shape[1], c[l - 1].shape[0])
+    return np.floor(np.clip(c[0] + 0.5, 0.0, hi)).astype(np.int64)   # (int)clamp(v + 0.5), v + 0.5 >= 0
+
+
+def _hybrid_restated(o, soft, hard, mix, start=2, count=4, hi=255, levels=None):
+    """mergeHybridImages src/fuse/pyramid.zig:271-301: laplacian pyramids of both images (reduceRgb
+    clamped, :418-487), L = soft + (hard - soft) * mix on levels [start, start + count), collapse."""
+    h, w = soft.shape[:2]
+    n = levels or len(level_dims(w, h))
+    ls, lh = _lap_pyramid(o, soft, n), _lap_pyramid(o, hard, n)
+    m = float(np.float32(mix))
+    hs = min(start, n)
+    he = min(hs + count, n)
+    mixed = [ls[l] + (lh[l] - ls[l]) * m if hs <= l < he else ls[l] for l in range(n)]
+    return _collapse_quantize(o, mixed, hi)
+
+
+@pytest.mark.parametrize("mix", [0.35, 1.0, 0.6])
+@pytest.mark.parametrize("h,w,dtype", [(110, 150, np.uint8), (37, 70, np.uint16), (700, 530, np.uint8)])
+def test_hybrid_merge_matches_numpy_restatement(oracle64, mix, h, w, dtype):
+    g = np.random.default_rng(h * 7 + w)
+    hi = np.iinfo(dtype).max
+    soft = g.integers(0, hi + 1, (h, w, 3)).astype(dtype)
+    hard = g.integers(0, hi + 1, (h, w, 3)).astype(dtype)
+    got = oracle64.merge_hybrid(soft, hard, mix).astype(np.int64)
+    ref = _hybrid_restated(oracle64, soft, hard, mix, hi=hi)
+    assert np.array_equal(got, ref)
+    # the slips this catches: operands swapped, the range shifted by one level either way
+    for wrong in (_hybrid_restated(oracle64, hard, soft, mix, hi=hi),
+                  _hybrid_restated(oracle64, soft, hard, mix, start=1, hi=hi),
+                  _hybrid_restated(oracle64, soft, hard, mix, start=3, hi=hi),
+                  _hybrid_restated(oracle64, soft, hard, mix, count=3, hi=hi)):
+        assert np.abs(wrong - got).max() > 1
+
+
+def test_hybrid_merge_sharpness_one_takes_the_hard_band(oracle64):
+    """At mix 1 levels [2, 6) are the hard image's own laplacian: a hard image that differs from the
+    soft one only in its band-pass content moves the output, one that differs only in its coarsest
+    levels (a constant offset) does not."""
+    g = np.random.default_rng(3)
+    soft = g.integers(40, 200, (96, 128, 3)).astype(np.uint8)
+    const = (soft.astype(int) + 17).astype(np.uint8)
+    out = oracle64.merge_hybrid(soft, const, 1.0)
+    assert np.abs(out.astype(int) - soft.astype(int)).max() <= 1
+    out_soft = oracle64.merge_hybrid(soft, g.integers(0, 256, soft.shape).astype(np.uint8), 1.0)
+    assert np.abs(out_soft.astype(int) - soft.astype(int)).max() > 8
+
+
+@pytest.mark.parametrize("levels", [3, 6, 7])
+def test_hybrid_merge_capped_levels_restated(oracle64, levels):
+    """With the pyramid capped the hard range is clipped to the levels that exist (:286-287)."""
+    g = np.random.default_rng(levels)
+    soft = g.integers(0, 256, (90, 120, 3)).astype(np.uint8)
+    hard = g.integers(0, 256, (90, 120, 3)).astype(np.uint8)
+    got = oracle64.merge_hybrid(soft, hard, 0.35, max_levels=levels).astype(np.int64)
+    assert np.array_equal(got, _hybrid_restated(oracle64, soft, hard, 0.35, levels=levels))
+
+
+@pytest.mark.parametrize("dtype", [np.uint8, np.uint16])
+def test_hybrid_stack_is_merge_of_pyramid_and_hardmask(oracle, dtype):
+    """src/stack/pipeline.zig:177-236: the hybrid's soft image is the pyramid-contrast result and its
+    hard image the hardmask-contrast result of the same frames; the output is their merge."""
+    from oracle import METHOD_PYRAMID, METHOD_HYBRID, METHOD_HARDMASK
+    frames = _frames(4, 52, 71, 21, dtype=dtype)
+    soft = oracle.fuse_frames(frames, method=METHOD_PYRAMID)
+    hard = oracle.fuse_frames(frames, method=METHOD_HARDMASK)
+    for s in (0.35, 1.0):
+        hyb, dbg = oracle.fuse_frames(frames, method=METHOD_HYBRID, sharpness=s, debug=True)
+        assert np.array_equal(dbg["soft"], soft) and np.array_equal(dbg["hard"], hard)
+        assert np.array_equal(hyb, oracle.merge_hybrid(soft, hard, s))
+
+
+def _alpha_frames(n, h, w, seed, dtype=np.uint8):
+    """Aligned frames whose alpha varies per frame (partial values, and holes in different places),
+    so that each frame's own support differs from the union support."""
+    g = np.random.default_rng(seed)
+    hi = np.iinfo(dtype).max
+    out = []
+    for k in range(n):
+        f = g.integers(0, hi + 1, (h, w, 4)).astype(dtype)
+        a = np.full((h, w), hi, np.int64)
+        a[: 3 + 2 * k, :] = 0                                   # top band, a different height per frame
+        a[:, w - 4 - k:] = g.integers(0, hi + 1, (h, 4 + k))    # right band of partial alpha
+        a[h // 2 - 5: h // 2 + 5, 7 * k: 7 * k + 9] = 0        # a hole at a different place per frame
+        f[..., 3] = a
+        out.append(f)
+    return out
+
+
+def _pyramid_blend_restated(o, frames, window=5, own_mask_support=False):
+    """accumulateImageWithWorkspace src/fuse/pyramid.zig:190-204: per frame, the normalised mask's
+    gaussian pyramid reduced with the UNION support at level 0 (:202, :384-416), the image's
+    gaussian pyramid reduced with the frame's OWN alpha > 0 support (:489-542), the laplacian times
+    the mask added level by level (:1506-1554); then the collapse (:223-269)."""
+    n = len(frames)
+    h, w = frames[0].shape[:2]
+    hi = np.iinfo(frames[0].dtype).max
+    dims = level_dims(w, h)
+    L = len(dims)
+    wts = []
+    for f in frames:
+        wk = o.contrast(o.gray(f), o.binary_support(f), window)
+        wts.append(wk * (f[..., 3].astype(np.float64) / hi))
+    S = np.zeros((h, w))
+    for wk in wts:
+        S = S + wk
+    U = np.zeros((h, w))
+    for f in frames:
+        U = np.maximum(U, (f[..., 3] > 0).astype(np.float64))
+    acc = [np.zeros((lh, lw, 3)) for (lw, lh) in dims]
+    for f, wk in zip(frames, wts):
+        m = [np.where(S > 1e-12, wk / np.where(S > 1e-12, S, 1.0), 1.0 / n)]
+        ms = [(f[..., 3] > 0).astype(np.float64) if own_mask_support else U]
+        for _ in range(1, L):
+            a, b = o.reduce_support(m[-1], ms[-1])
+            m.append(a)
+            ms.append(b)
+        g = [f[..., :3].astype(np.float64)]
+        gs = [(f[..., 3] > 0).astype(np.float64)]
+        for _ in range(1, L):
+            a, b = o.reduce_support(g[-1], gs[-1])
+            g.append(a)
+            gs.append(b)
+        for l in range(L - 1):
+            lap = g[l] - o.expand(g[l + 1], dims[l][0], dims[l][1])
+            acc[l] = acc[l] + lap * m[l][..., None]
+        acc[L - 1] = acc[L - 1] + g[L - 1] * m[L - 1][..., None]
+    return _collapse_quantize(o, acc, hi)
+
+
+@pytest.mark.parametrize("h,w,dtype", [(48, 64, np.uint8), (61, 45, np.uint16), (600, 560, np.uint8)])
+def test_masked_pyramid_blend_matches_numpy_restatement(oracle64, h, w, dtype):
+    frames = _alpha_frames(3, h, w, h + w, dtype)
+    got = oracle64.fuse_frames(frames, extra=1).astype(np.int64)
+    assert np.array_equal(got, _pyramid_blend_restated(oracle64, frames))
+    # the slip this catches: the frame's own support in place of the union for the mask pyramid
+    wrong = _pyramid_blend_restated(oracle64, frames, own_mask_support=True)
+    assert np.abs(wrong - got).max() > 1
