@@ -1,16 +1,19 @@
 """Benchmark: remap + pyramid-contrast fusion of one synthetic stack per step on one MI355X.
 
-Workload (N=1 line): 30 frames of 7952x5304 RGB u8 from the seeded synthetic generator
-(band-limited value-noise scene, depth ramp + Gaussian bump, per-frame defocus blur sigma up
-to 6 px, similarity poses with shifts U(+-20) px, roll U(+-0.3) deg, scale 1+U(+-0.01)),
-remapped into the common-overlap ROI and fused with the reference's full pyramid depth.
+Workload (N=1 line, --config 2, the default): the streaming large stack, 143 frames of
+7952x5304 RGB u16 (12-bit values scaled to 16 bit) from the seeded synthetic generator
+(band-limited value-noise scene, depth ramp + three Gaussian bumps, per-frame defocus blur sigma
+up to 6 px, similarity poses with shifts U(+-20) px, roll U(+-0.3) deg, scale 1+U(+-0.01)),
+remapped into the common-overlap ROI and fused with the reference's full pyramid depth into
+device-resident FP32 per-level accumulators.  --config 1 is the 30-frame u8 stack.
 Frames are generated on the GPU and stay resident in HBM; the timed region is K steps of
-{reset, 30 x add_frame_device (remap + pass 1), finish_device (pass 2 + collapse)}.
+{reset, N x add_frame_device (remap + pass 1), finish_device (pass 2 + collapse)}.
 
 Output: one JSON line.  roofline: the longer of the two dominant launches (k_pass1, the level-0
 k_pass2), its per frame-pixel algorithmic bytes (kernel_models) times the frame-pixels of one
 launch over its mean duration from HIP events on the stack's stream in the timed region.
-cpu_baseline: the C oracle on 16 host threads over a bounded sample of the same stack.
+cpu_baseline: the C oracle (the reference's algorithm restated, its independent loops spread over
+every core the job has) fusing the first two frames of the same workload as a full-size stack.
 """
 import argparse
 import json
@@ -43,9 +46,12 @@ def kernel_models(s: int) -> dict:
     return {"k_pass1": ("pass1", 3 * s + 14), "k_pass2_level0": ("pass2_level0", 3 * s + 33)}
 
 
-# HBM bytes per launch of the default workload, from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE
-# runs scaled by the same counters on a plain copy (profiles/pmc_traffic_30x7952x5304_u8.md)
-TRAFFIC = {"k_pass1": 20.263e9, "k_pass2_level0": 28.988e9}
+# HBM bytes per launch, from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs scaled by the same
+# counters on a plain copy of known size (profiles/pmc_traffic_<workload>.md), keyed by
+# (frames, width, height, samples, method, levels)
+TRAFFIC = {
+    (30, 7952, 5304, "u8", "pyramid", 0): {"k_pass1": 20.263e9, "k_pass2_level0": 28.988e9},
+}
 COPY_PROBE_BYTES = 256 << 20
 
 
@@ -57,38 +63,35 @@ def pct(v, q):
     return v[k]
 
 
-CPU_THREADS = 16  # the host-core share of one GPU on the measurement boxes
+def job_cores() -> int:
+    """Cores this job may use: OMP_NUM_THREADS when set (the measurement boxes set it to the
+    host-core share of one GPU), else the CPUs in the process's affinity mask."""
+    n = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+    return n if n > 0 else len(os.sched_getaffinity(0))
 
 
-def cpu_baseline(frames_np, poses, roi, method=0, threads=CPU_THREADS):
-    """The oracle (the C restatement) on a bounded sample of the same workload, on `threads` host
-    cores: frames 0-1 at full size, the ROI cut into `threads` horizontal bands, each fused as its
-    own stack by one thread (ctypes releases the GIL for the call).  A band's pyramid is one or two
-    levels shallower than the full ROI's; the levels it lacks hold under 0.1% of the pixels."""
-    from concurrent.futures import ThreadPoolExecutor
+def cpu_baseline(frames_np, poses, roi, method=0, max_levels=0):
+    """The oracle (the C restatement of the reference's algorithm, its independent per-pixel and
+    per-row loops spread over every core of the job, as the reference spreads row blocks over its
+    --threads) on a bounded sample of the same workload: the first two frames fused as a stack over
+    the whole ROI, the same per frame-pixel work as every frame of the full stack (remap, weights,
+    norm sum, both pyramids, accumulation; the collapse once)."""
     from oracle import Oracle, make_pose
     o = Oracle()
+    cores = job_cores()
+    o.set_threads(cores)
     l, t, r, b = roi
-    cuts = [t + (b - t) * i // threads for i in range(threads + 1)]
-    bands = [(l, cuts[i], r, cuts[i + 1]) for i in range(threads)]
     src, opo = frames_np[:2], [make_pose(**p) for p in poses[:2]]
-
-    def run(band):
-        t0 = time.perf_counter()
-        o.stack(src, opo, band, method=method)
-        return time.perf_counter() - t0
-
+    c0 = time.process_time()
     t0 = time.perf_counter()
-    with ThreadPoolExecutor(threads) as ex:
-        per_band = list(ex.map(run, bands))
+    o.stack(src, opo, roi, method=method, max_levels=max_levels)
     dt = time.perf_counter() - t0
-    px = 2 * (r - l) * (b - t)
-    one = px / threads / (sum(per_band) / threads) / 1e6
-    return {"value": round(px / dt / 1e6, 3), "unit": "MPix/s", "cores": threads, "kind": "port",
-            "sample": f"oracle/focus_oracle.c, frames 0-1 of the workload at full size over the whole ROI "
-                      f"({r - l}x{b - t}), {threads} bands of about {(b - t) // threads} rows, one thread each, "
-                      f"remap+weights+pyramid+collapse, {dt:.1f} s wall, {sum(per_band):.1f} s of CPU work, "
-                      f"{one:.2f} MPix/s per thread"}
+    cpu_s = time.process_time() - c0
+    px = len(src) * (r - l) * (b - t)
+    return {"value": round(px / dt / 1e6, 3), "unit": "MPix/s", "cores": cores, "kind": "port",
+            "sample": f"oracle/focus_oracle.c fusing frames 0-1 of the workload as a 2-frame stack over the whole "
+                      f"ROI ({r - l}x{b - t}) on {cores} threads: remap + weights + pyramids + accumulation + "
+                      f"collapse, {dt:.1f} s wall, {cpu_s:.1f} s of CPU time"}
 
 
 MIX_SIZES = [(640, 480), (1920, 1080), (4000, 3000), (6000, 4000)]
@@ -245,11 +248,11 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-host", action="store_true", help="skip the host-fed (PCIe-inclusive) step")
     ap.add_argument("--profile-only", action="store_true", help="one profiled step (for rocprofv3 runs)")
-    ap.add_argument("--config", type=int, choices=(1, 2), default=1,
-                    help="1: 30 x 7952x5304 u8 (the reported line); 2: 143 x 7952x5304 u16, three focal planes")
+    ap.add_argument("--config", type=int, choices=(1, 2), default=2,
+                    help="2: 143 x 7952x5304 u16, three focal planes (the reported line); 1: 30 x 7952x5304 u8")
     ap.add_argument("--method", choices=("pyramid", "hybrid", "hardmask", "softmask"), default="pyramid")
     ap.add_argument("--workload", choices=("stack", "batch", "serve"), default="stack",
-                    help="stack: the 30 x 7952x5304 stack (default); batch / serve: uneven stack mixes")
+                    help="stack: one large stack per step (default); batch / serve: uneven stack mixes")
     ap.add_argument("--stacks", type=int, default=64)
     ap.add_argument("--slots", type=int, default=4)
     ap.add_argument("--requests", type=int, default=48)
@@ -368,8 +371,7 @@ def main():
     pipe_bytes = alg_bytes_per_frame_px(s_b, n) * frame_px
     pipe_gbs = pipe_bytes / (ms_per_step / 1000.0) / 1e9
     cfg_name = f"{n} x {spec.width}x{spec.height} RGB {tname}"
-    traffic = (TRAFFIC.get(dom) if (n, spec.width, spec.height, args.levels, tname, args.method) ==
-               (30, 7952, 5304, 0, "u8", "pyramid") else None)
+    traffic = TRAFFIC.get((n, spec.width, spec.height, tname, args.method, args.levels), {}).get(dom)
 
     line = {
         "metric": METRIC,
@@ -382,10 +384,12 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": tname,
+        "dtype": "fp32",
         "data": "synthetic",
         "config": {"workload": f"{cfg_name} stack (synthetic, seeded), remap + {args.method}-contrast, {levels} levels",
-                   "frames": n, "source": [spec.width, spec.height], "roi": list(roi), "levels": levels,
+                   "frames": n, "source": [spec.width, spec.height], "samples": tname, "roi": list(roi),
+                   "levels": levels, "focal_planes": spec.planes,
+                   "arithmetic": "fp32 pyramids, masks and accumulators; f64 remap coordinates and contrast sums",
                    "parallelism": "single GPU"},
         "latency_ms": {"p50": round(pct(lat, 50), 3), "p99": round(pct(lat, 99), 3)},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": kern[dom]["achieved"], "peak": PEAK_HBM_GBS,
@@ -399,6 +403,7 @@ def main():
     if not args.no_host:
         # the same stack fed from pageable host memory through fg_stack_add_frame: PCIe-inclusive
         # (uploads overlap pass 1 of earlier frames on the stack's upload stream)
+        print("bench: host-fed steps", file=sys.stderr, flush=True)
         frames_host = [f.cpu().numpy() for f in frames]
         hsteps = 2
         stack.reset()
@@ -419,13 +424,13 @@ def main():
                                   "ms_per_step": round(hms, 3), "source_GB_per_step": round(src_gb, 3),
                                   "h2d_GBs": round(src_gb / (hms / 1000.0), 2),
                                   "note": "frames in pageable host memory, fg_stack_add_frame; not the value"}
-        del frames_host
         # the same from page-locked buffers (fg_host_alloc), as a decoder writing into them would
         pinned = []
-        for f in frames:
-            pa = fg.PinnedArray(tuple(f.shape), np.uint16 if u16 else np.uint8)
-            pa.array[...] = f.cpu().numpy()
+        for fh in frames_host:
+            pa = fg.PinnedArray(tuple(fh.shape), np.uint16 if u16 else np.uint8)
+            pa.array[...] = fh.view(np.uint16) if u16 else fh
             pinned.append(pa)
+        del frames_host
         t0 = time.perf_counter()
         for _ in range(hsteps):
             stack.reset()
@@ -445,7 +450,7 @@ def main():
         print("bench: timing the CPU baseline (oracle) on a bounded sample", file=sys.stderr, flush=True)
         frames_np = [f.cpu().numpy().view(np.uint16) if u16 else f.cpu().numpy() for f in frames[:2]]
         omethod = {"pyramid": 0, "hybrid": 1, "hardmask": 2, "softmask": 3}[args.method]
-        line["cpu_baseline"] = cpu_baseline(frames_np, poses, roi, omethod)
+        line["cpu_baseline"] = cpu_baseline(frames_np, poses, roi, omethod, args.levels)
     print(json.dumps(line), flush=True)
 
 

@@ -353,6 +353,30 @@ __device__ __forceinline__ bool bilinear_taps_u8(const uint32_t t[12], double fx
     return near ? bilinear_taps(t, fx, fy, band, o) : true;
 }
 
+// u16 interior taps in f64 lerp form on exact integer differences, without the weights' sum and
+// its reciprocal: top = p00 + fx (p10 - p00), bottom likewise, value = top + fy (bottom - top).
+// Against the reference's sum(p w) / sum(w) (weights rounded in f64, sum(w) = 1 to within a few
+// ulp) this differs by a few ulp of 65535, about 1e-10; `band` (4e-6, set by the reciprocal
+// coordinates) is far wider, so outside it both round the same way, and inside it the pixel goes
+// to the exact path.
+__device__ __forceinline__ bool bilinear_taps_lerp64(const uint32_t t[12], double fx, double fy, double band,
+                                                     uint32_t o[3]) {
+    bool ok = true;
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        const double p00 = (double)t[c], p01 = (double)t[6 + c];
+        const double d0 = (double)((int)t[3 + c] - (int)t[c]), d1 = (double)((int)t[9 + c] - (int)t[6 + c]);
+        const double top = fma(fx, d0, p00);
+        const double bot = fma(fx, d1, p01);
+        const double q = fma(fy, bot - top, top) + 0.5;
+        const double fl = floor(q);
+        const double d = q - fl;
+        ok = ok && fabs(d - 0.5) < 0.5 - band;
+        o[c] = (uint32_t)fl;
+    }
+    return ok;
+}
+
 // Region pixel in row gy of the homography fast path: coordinate by one reciprocal
 // (fast_coord), interior taps from the staged box (or from global memory when the frame is not
 // staged).  False where the exact path has to decide.  STAGED is the block-uniform choice between
@@ -404,13 +428,15 @@ __device__ __forceinline__ bool remap_fast(const FastFrame& ff, const ColHomog& 
         }
     }
     if (sizeof(T) == 1) return bilinear_taps_u8(t, x - fx0, y - fy0, band, o);
-    return bilinear_taps(t, x - fx0, y - fy0, band, o);
+    return bilinear_taps_lerp64(t, x - fx0, y - fy0, band, o);
 }
 
 // Cross-lane moves within a wave (DPP wave_shr:1 / wave_shl:1): the value of lane - 1 / lane + 1,
-// 0 at the edge lane.  Every lane of the wave must execute them.
-__device__ __forceinline__ int lane_prev(int v) { return __builtin_amdgcn_update_dpp(0, v, 0x138, 0xf, 0xf, false); }
-__device__ __forceinline__ int lane_next(int v) { return __builtin_amdgcn_update_dpp(0, v, 0x130, 0xf, 0xf, false); }
+// 0 at the edge lane (bound_ctrl: an out-of-range source reads 0, so no old value is needed).
+// Every lane of the wave must execute them.  The window sums below only feed lanes b..63-b, whose
+// windows never leave the wave, so the edge value is never used.
+__device__ __forceinline__ int lane_prev(int v) { return __builtin_amdgcn_mov_dpp(v, 0x138, 0xf, 0xf, true); }
+__device__ __forceinline__ int lane_next(int v) { return __builtin_amdgcn_mov_dpp(v, 0x130, 0xf, 0xf, true); }
 __device__ __forceinline__ double lane_prev(double v) {
     return __hiloint2double(lane_prev(__double2hiint(v)), lane_prev(__double2loint(v)));
 }
@@ -487,6 +513,7 @@ __global__ void __launch_bounds__(256, sizeof(T) == 1 ? 4 : 3) k_pass1(Pass1Args
     float* tB = (float*)(smem + L.tB);                // [th][64] hybrid: best weight
     uint16_t* tK = (uint16_t*)(smem + L.tK);          // [th][64] hybrid, frames cached: its slot
     __shared__ double corners[8];                     // next frame's region corners in its source
+    __shared__ int sFull[4];                          // per wave: every region pixel it remapped has full alpha
 
     const int W = (int)a.w, H = (int)a.h;
     const int X0 = blockIdx.x * tw, Y0 = blockIdx.y * th;
@@ -575,6 +602,9 @@ __global__ void __launch_bounds__(256, sizeof(T) == 1 ? 4 : 3) k_pass1(Pass1Args
         // remap the region (remapRigidImage); gray (fillAverageFromLoaded) where alpha > 0.  The
         // loop is compiled once per tap source (staged box or global gather), box.ok being the
         // same for the whole block.
+        // every region pixel inside the stack and at full alpha (a wave vote after the remap): the
+        // contrast then needs no support tests and every window holds window^2 samples
+        bool lane_full = true;
         auto remap_rows = [&](auto staged) {
 #pragma unroll 1
             for (int r = wave; r < RH; r += 4) {
@@ -608,102 +638,133 @@ __global__ void __launch_bounds__(256, sizeof(T) == 1 ? 4 : 3) k_pass1(Pass1Args
                 gray[r * P1C + lane] = g;
                 alph[r * P1C + lane] = (AT)o[3];
                 if (COL) colour[r * P1C + lane] = c;
+                lane_full = lane_full && o[3] == SampleMax<T>::i;
             }
         };
         if (box.ok)
             remap_rows(std::true_type{});
         else
             remap_rows(std::false_type{});
+        {
+            const bool wf = __all(lane_full ? 1 : 0);
+            if (lane == 0) sFull[wave] = wf ? 1 : 0;
+        }
         corner(k + 1);
         lds_barrier();
+        const bool full = contrast_ok && sFull[0] && sFull[1] && sFull[2] && sFull[3];
         // the next frame's source box, in flight until the end of this frame
         StageBox nbox = k + 1 < a.k1 ? stage_box<T>(corners, (int)a.frames[k + 1].sw, (int)a.frames[k + 1].sh)
                                      : StageBox{0, 0, 0, 0, 0};
         if (nbox.ok) stage_load<T>(a.frames[k + 1], nbox, lane, wave, pf);
         // contrast of this wave's output rows: window-column sums slide down the column in
         // registers (lane = column), the window's columns are added across lanes
-        // (computeRange src/fuse/contrast.zig:172-282: f64 sums of x and of the f32 product x*x)
-        double vs = 0.0, vq = 0.0;
-        int vn = 0;
+        // (computeRange src/fuse/contrast.zig:172-282: f64 sums of x and of the f32 product x*x).
+        // Compiled for the default window 5 (BW = 2) and for any window (BW = 0: b at run time),
+        // each with and without the fully supported region (FULL: every sample counts, so the
+        // count is window^2 and alpha is full; the sums take the same operations either way).
+        auto contrast_rows = [&](auto bw, auto full_c) {
+            constexpr int BW = decltype(bw)::value;
+            constexpr bool FULL = decltype(full_c)::value;
+            const int bb = BW > 0 ? BW : b;
+            auto wsum = [&](auto x) {
+                if constexpr (BW > 0)
+                    return lane_window_sum_b<BW>(x);
+                else
+                    return lane_window_sum(x, bb);
+            };
+            double vs = 0.0, vq = 0.0;
+            int vn = 0;
 #pragma unroll 1
-        for (int r = r0; r < r1; r++) {
-            if (contrast_ok) {
-                if (r == r0) {
-                    for (int dy = 0; dy <= 2 * b; dy++) {
-                        const float v = gray[(r0 + dy) * P1C + lane];
-                        if (v >= 0.0f) {
-                            vs += (double)v;
-                            vq += (double)(v * v);
+            for (int r = r0; r < r1; r++) {
+                if (contrast_ok) {
+                    if (r == r0) {
+                        for (int dy = 0; dy <= 2 * bb; dy++) {
+                            const float v = gray[(r0 + dy) * P1C + lane];
+                            if (FULL || v >= 0.0f) {
+                                vs += (double)v;
+                                vq += (double)(v * v);
+                                vn++;
+                            }
+                        }
+                    } else {
+                        const float vi = gray[(r + 2 * bb) * P1C + lane];
+                        const float vo = gray[(r - 1) * P1C + lane];
+                        if (FULL || vi >= 0.0f) {
+                            vs += (double)vi;
+                            vq += (double)(vi * vi);
                             vn++;
                         }
+                        if (FULL || vo >= 0.0f) {
+                            vs -= (double)vo;
+                            vq -= (double)(vo * vo);
+                            vn--;
+                        }
                     }
-                } else {
-                    const float vi = gray[(r + 2 * b) * P1C + lane];
-                    const float vo = gray[(r - 1) * P1C + lane];
-                    if (vi >= 0.0f) {
-                        vs += (double)vi;
-                        vq += (double)(vi * vi);
-                        vn++;
+                }
+                double sw = 0.0, qw = 0.0;
+                int n = FULL ? nfull : 0;
+                if (contrast_ok) {  // every lane takes part in the cross-lane sums
+                    sw = wsum(vs);
+                    qw = wsum(vq);
+                    if (!FULL) n = wsum(vn);
+                }
+                const int y = Y0 + r;
+                if (umask) {  // accumulateBinarySupportMax as one lane-mask OR per row (hybrid layout)
+                    const uint64_t um = __ballot(col_ok && y < H && (FULL || alph[(r + bb) * P1C + lane] > 0));
+                    if (lane == 0) atomicOr((unsigned long long*)&tUw[r], (unsigned long long)um);
+                }
+                if (!col_ok || y >= H) continue;
+                const size_t i = (size_t)y * W + ox;
+                float stdw = 0.0f;
+                if (contrast_ok && ox >= bb && ox < W - bb && y >= bb && y < H - bb &&
+                    (FULL || (gray[(r + bb) * P1C + lane] >= 0.0f && n > 1))) {  // sampleStdDev :284-290
+                    double m, var;
+                    if (FULL || n == nfull) {
+                        m = div_markstein(sw * sw, nf_full, r_full);
+                        var = div_markstein(qw - m, nf_full1, r_full1);
+                    } else {
+                        const double nf = (double)n, den = (double)(n - 1);
+                        m = (sw * sw) / nf;
+                        var = (qw - m) / den;
                     }
-                    if (vo >= 0.0f) {
-                        vs -= (double)vo;
-                        vq -= (double)(vo * vo);
-                        vn--;
+                    if (!(var > 0.0)) var = 0.0;
+                    stdw = (float)sqrt(var);
+                }
+                const uint32_t alpha = FULL ? SampleMax<T>::i : (uint32_t)alph[(r + bb) * P1C + lane];
+                const float af = FULL || alpha == SampleMax<T>::i
+                                     ? 1.0f
+                                     : div_markstein((float)alpha, SampleMax<T>::f, 1.0f / SampleMax<T>::f);
+                const float wv = stdw * af;  // applySupportInto
+                if (keep_w) wout[i] = wv;
+                const int t = r * P1C + lane;
+                if (acc) {
+                    tS[t] = tS[t] + wv;
+                    if (!umask && alpha > 0) tUb[t] = 1;
+                }
+                if (hyb && wv > tB[t]) {  // updateRange: strictly greater wins, first frame on ties
+                    tB[t] = wv;
+                    if (!COL) {
+                        tK[t] = (uint16_t)slot;
+                    } else {  // only this thread writes this pixel of the hard image
+                        const PT c = colour[(r + bb) * P1C + lane];
+                        T* hd = (T*)a.hard + i * 3;
+                        hd[0] = (T)P::ch(c, 0);
+                        hd[1] = (T)P::ch(c, 1);
+                        hd[2] = (T)P::ch(c, 2);
                     }
                 }
             }
-            double s = 0.0, q = 0.0;
-            int n = 0;
-            if (contrast_ok) {  // every lane takes part in the cross-lane sums
-                s = lane_window_sum(vs, b);
-                q = lane_window_sum(vq, b);
-                n = lane_window_sum(vn, b);
-            }
-            const int y = Y0 + r;
-            if (umask) {  // accumulateBinarySupportMax as one lane-mask OR per row (hybrid layout)
-                const uint64_t um = __ballot(col_ok && y < H && alph[(r + b) * P1C + lane] > 0);
-                if (lane == 0) atomicOr((unsigned long long*)&tUw[r], (unsigned long long)um);
-            }
-            if (!col_ok || y >= H) continue;
-            const size_t i = (size_t)y * W + ox;
-            float stdw = 0.0f;
-            if (contrast_ok && ox >= b && ox < W - b && y >= b && y < H - b && gray[(r + b) * P1C + lane] >= 0.0f &&
-                n > 1) {  // sampleStdDev src/fuse/contrast.zig:284-290
-                double m, var;
-                if (n == nfull) {
-                    m = div_markstein(s * s, nf_full, r_full);
-                    var = div_markstein(q - m, nf_full1, r_full1);
-                } else {
-                    const double nf = (double)n, den = (double)(n - 1);
-                    m = (s * s) / nf;
-                    var = (q - m) / den;
-                }
-                if (!(var > 0.0)) var = 0.0;
-                stdw = (float)sqrt(var);
-            }
-            const uint32_t alpha = alph[(r + b) * P1C + lane];
-            const float af = alpha == SampleMax<T>::i
-                                 ? 1.0f
-                                 : div_markstein((float)alpha, SampleMax<T>::f, 1.0f / SampleMax<T>::f);
-            const float wv = stdw * af;  // applySupportInto
-            if (keep_w) wout[i] = wv;
-            const int t = r * P1C + lane;
-            if (acc) {
-                tS[t] = tS[t] + wv;
-                if (!umask && alpha > 0) tUb[t] = 1;
-            }
-            if (hyb && wv > tB[t]) {  // updateRange: strictly greater wins, first frame on ties
-                tB[t] = wv;
-                if (!COL) {
-                    tK[t] = (uint16_t)slot;
-                } else {  // only this thread writes this pixel of the hard image
-                    const PT c = colour[(r + b) * P1C + lane];
-                    T* hd = (T*)a.hard + i * 3;
-                    hd[0] = (T)P::ch(c, 0);
-                    hd[1] = (T)P::ch(c, 1);
-                    hd[2] = (T)P::ch(c, 2);
-                }
-            }
+        };
+        if (b == 2) {
+            if (full)
+                contrast_rows(std::integral_constant<int, 2>{}, std::true_type{});
+            else
+                contrast_rows(std::integral_constant<int, 2>{}, std::false_type{});
+        } else {
+            if (full)
+                contrast_rows(std::integral_constant<int, 0>{}, std::true_type{});
+            else
+                contrast_rows(std::integral_constant<int, 0>{}, std::false_type{});
         }
         // the next frame's box: its loads were issued before the contrast, the remap of this
         // frame is done with the staging (first barrier above)

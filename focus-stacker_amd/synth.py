@@ -148,18 +148,24 @@ def make_depth(spec: StackSpec, device="cpu") -> torch.Tensor:
 
 
 def _gauss_blur(img: torch.Tensor, sigma: float) -> torch.Tensor:
+    """Separable Gaussian (radius 3 sigma, replicated edges) as a sum of shifted copies: plain
+    elementwise work, with no convolution library and its first-use set-up on the device."""
     if sigma <= 0:
         return img
     r = int(math.ceil(3 * sigma))
-    x = torch.arange(-r, r + 1, device=img.device, dtype=torch.float32)
-    k = torch.exp(-x * x / (2 * sigma * sigma))
-    k = k / k.sum()
-    c = img.shape[0]
-    t = F.pad(img[None], (r, r, 0, 0), mode="replicate")
-    t = F.conv2d(t, k.view(1, 1, 1, -1).repeat(c, 1, 1, 1), groups=c)
-    t = F.pad(t, (0, 0, r, r), mode="replicate")
-    t = F.conv2d(t, k.view(1, 1, -1, 1).repeat(c, 1, 1, 1), groups=c)
-    return t[0]
+    x = np.arange(-r, r + 1, dtype=np.float64)
+    k = np.exp(-x * x / (2 * sigma * sigma))
+    k = (k / k.sum()).astype(np.float32)
+    h, w = img.shape[1], img.shape[2]
+    t = F.pad(img[None], (r, r, 0, 0), mode="replicate")[0]
+    acc = t[:, :, 0:w] * float(k[0])
+    for j in range(1, 2 * r + 1):
+        acc += t[:, :, j:j + w] * float(k[j])
+    t = F.pad(acc[None], (0, 0, r, r), mode="replicate")[0]
+    acc = t[:, 0:h, :] * float(k[0])
+    for j in range(1, 2 * r + 1):
+        acc += t[:, j:j + h, :] * float(k[j])
+    return acc
 
 
 class Stack:

@@ -29,7 +29,15 @@ def test_dominant_kernel_models_fit_the_pipeline(bench):
         m = bench.kernel_models(s)
         assert m["k_pass1"][1] == 3 * s + 14
         assert m["k_pass2_level0"][1] == (3 * s + 41) - 8
-        assert set(m) == set(bench.TRAFFIC)
+        for per_launch in bench.TRAFFIC.values():
+            assert set(per_launch) == set(m)
+
+
+def test_job_cores_follows_omp_num_threads(bench, monkeypatch):
+    monkeypatch.setenv("OMP_NUM_THREADS", "5")
+    assert bench.job_cores() == 5
+    monkeypatch.delenv("OMP_NUM_THREADS")
+    assert bench.job_cores() == len(os.sched_getaffinity(0))
 
 
 def test_percentile(bench):
