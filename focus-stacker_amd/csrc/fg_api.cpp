@@ -695,7 +695,7 @@ struct StackPlan {
     size_t bytes;
 };
 
-static StackPlan plan_stack(const fg_stack_opts& o, double budget) {
+static StackPlan plan_stack(const fg_stack_opts& o, double budget, bool host_frames = true) {
     StackPlan pl{};
     const size_t sb = o.sample_type == FG_SAMPLE_U16 ? 2 : 1;
     fg::make_levels(o.width, o.height, o.max_levels, &pl.lt);
@@ -715,7 +715,7 @@ static StackPlan plan_stack(const fg_stack_opts& o, double budget) {
     // frames fed from host memory go through an upload ring of kStageSlots source frames, which
     // describe_frame allocates at the first such frame; the caching decision leaves room for it
     // (sources the size of the fused area; a larger source makes the ring shrink, not fail)
-    const double ring = (double)kStageSlots * (double)pl.px * 3.0 * (double)sb;
+    const double ring = host_frames ? (double)kStageSlots * (double)pl.px * 3.0 * (double)sb : 0.0;
     pl.cached = !pl.single && (double)fixed + ring + (double)o.frame_count * (double)per_frame <= budget;
     pl.slots = pl.cached ? o.frame_count : 1;
     // single-pass methods keep no frame past its group: softmask holds a group of frames
@@ -740,15 +740,15 @@ static int check_stack_opts(const fg_stack_opts* o) {
 int fg_context_device(const fg_context* ctx) { return ctx ? ctx->device : 0; }
 int fg_set_error(int code, const char* msg) { return fail(code, msg ? msg : ""); }
 
-size_t fg_plan_stack_bytes(const fg_stack_opts* o, double budget, int* cached) {
+size_t fg_plan_stack_bytes(const fg_stack_opts* o, double budget, int* cached, bool host_frames) {
     if (check_stack_opts(o)) return 0;
-    StackPlan pl = plan_stack(*o, budget);
+    StackPlan pl = plan_stack(*o, budget, host_frames);
     if (cached) *cached = pl.cached ? 1 : 0;
     return pl.bytes;
 }
 
 int fg_stack_create_budget(fg_context* ctx, const fg_stack_opts* o, double budget, fg_stack** out,
-                           const fg_dev_allocator* allocator) {
+                           const fg_dev_allocator* allocator, bool host_frames) {
     if (!ctx || !o || !out) return fail(FG_ERR_INVALID_ARG, "null argument");
     *out = nullptr;
     int rc = check_stack_opts(o);
@@ -761,7 +761,7 @@ int fg_stack_create_budget(fg_context* ctx, const fg_stack_opts* o, double budge
         float frac = o->memory_fraction >= 0.0f ? o->memory_fraction : ctx->memory_fraction;
         budget = frac > 0.0f ? (double)frac * (double)free_b : (double)free_b - 1.0e9;
     }
-    const StackPlan pl = plan_stack(*o, budget);
+    const StackPlan pl = plan_stack(*o, budget, host_frames);
     fg_stack* s = new (std::nothrow) fg_stack();
     if (!s) return fail(FG_ERR_OOM_HOST, "stack allocation");
     s->ctx = ctx;
@@ -851,6 +851,7 @@ void fg_stack_destroy_idle(fg_stack* s) {
 }
 
 float fg_context_memory_fraction(const fg_context* ctx) { return ctx ? ctx->memory_fraction : 0.0f; }
+void* fg_context_stream(const fg_context* ctx) { return ctx ? (void*)ctx->stream : nullptr; }
 
 extern "C" {
 

@@ -6,8 +6,9 @@
 #include "../../include/focus_gpu.h"
 
 // Device bytes of a stack with these options when `budget` bytes are available; *cached tells
-// whether every frame then stays on the device.  0 for invalid options.
-size_t fg_plan_stack_bytes(const fg_stack_opts* o, double budget, int* cached);
+// whether every frame then stays on the device.  0 for invalid options.  host_frames: frames may
+// come from host memory, so the caching decision leaves room for the upload ring.
+size_t fg_plan_stack_bytes(const fg_stack_opts* o, double budget, int* cached, bool host_frames = true);
 // Device memory source for a stack's buffers (the server's arena): alloc returns nullptr when it
 // cannot serve the request; free takes back what alloc gave.
 struct fg_dev_allocator {
@@ -18,10 +19,12 @@ struct fg_dev_allocator {
 // fg_stack_create with the caching decision taken against `budget` bytes (< 0: the options'
 // memory fraction of free HBM) and its buffers from `allocator` (nullptr: hipMalloc).
 int fg_stack_create_budget(fg_context* ctx, const fg_stack_opts* o, double budget, fg_stack** out,
-                           const fg_dev_allocator* allocator = nullptr);
+                           const fg_dev_allocator* allocator = nullptr, bool host_frames = true);
 // HIP device of a context, and its --memory-fraction.
 int fg_context_device(const fg_context* ctx);
 float fg_context_memory_fraction(const fg_context* ctx);
+// The context's own stream (hipStream_t).
+void* fg_context_stream(const fg_context* ctx);
 // fg_stack_destroy for a stack whose work is known complete: does not wait on its stream (which
 // may be shared with, and busy running, another stack).
 void fg_stack_destroy_idle(fg_stack* stack);

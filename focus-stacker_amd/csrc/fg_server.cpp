@@ -196,10 +196,10 @@ void fg_server::start(int slot, const std::shared_ptr<Request>& r) {
         o.debug_frame = -1;
         o.memory_fraction = 0.0f;
         const double left = budget > in_use ? budget - in_use : 0.0;
-        rc = fg_stack_create_budget(ctx, &o, left, &st, &hook);
+        rc = fg_stack_create_budget(ctx, &o, left, &st, &hook, false);
         if (rc == FG_ERR_OOM_DEVICE) {  // a fragmented arena: give back the idle stacks and retry
             drop_idle();
-            rc = fg_stack_create_budget(ctx, &o, left, &st, &hook);
+            rc = fg_stack_create_budget(ctx, &o, left, &st, &hook, false);
         }
         if (rc == FG_ERR_OOM_DEVICE) {
             bool others = false;
@@ -213,7 +213,7 @@ void fg_server::start(int slot, const std::shared_ptr<Request>& r) {
             // alone: a stack larger than the whole budget still runs (uncached, as the
             // reference does when its frame cache does not fit)
             arena.overflow = true;
-            rc = fg_stack_create_budget(ctx, &o, left, &st, &hook);
+            rc = fg_stack_create_budget(ctx, &o, left, &st, &hook, false);
             arena.overflow = false;
         }
         if (rc == FG_OK) in_use += (double)r->bytes;
@@ -291,7 +291,7 @@ bool fg_server::admit(std::unique_lock<std::mutex>&) {
         for (const Pooled& p : idle[slot]) pooled = pooled || same_geometry(p.key, r->opts);
         if (!pooled) {
             int cached = 0;
-            size_t need = fg_plan_stack_bytes(&r->opts, budget > in_use ? budget - in_use : 0.0, &cached);
+            size_t need = fg_plan_stack_bytes(&r->opts, budget > in_use ? budget - in_use : 0.0, &cached, false);
             // free idle stacks (least recently used first) until it fits
             while (in_use + (double)need > budget) {
                 int bs = -1;
@@ -306,7 +306,7 @@ bool fg_server::admit(std::unique_lock<std::mutex>&) {
                 fg_stack_destroy_idle(idle[bs][bi].stack);
                 in_use -= (double)idle[bs][bi].bytes;
                 idle[bs].erase(idle[bs].begin() + (long)bi);
-                need = fg_plan_stack_bytes(&r->opts, budget > in_use ? budget - in_use : 0.0, &cached);
+                need = fg_plan_stack_bytes(&r->opts, budget > in_use ? budget - in_use : 0.0, &cached, false);
             }
             bool busy = false;
             for (int i = 0; i < nslots; i++) busy = busy || running[i];
