@@ -610,6 +610,9 @@ int fg_context_create(const fg_context_opts* opts, fg_context** out) {
         delete c;
         return fail_hip(e, "hipStreamCreate");
     }
+    // the device address of this GPU's scratch area for fg_fuse_stack_device_noalloc (a failure
+    // here only disables that entry point, which then reports it)
+    fg_noalloc_setup(dev);
     *out = c;
     return FG_OK;
 }

@@ -25,6 +25,9 @@ int fg_context_device(const fg_context* ctx);
 float fg_context_memory_fraction(const fg_context* ctx);
 // The context's own stream (hipStream_t).
 void* fg_context_stream(const fg_context* ctx);
+// Takes the device address of this GPU's scratch area for fg_fuse_stack_device_noalloc, once per
+// device, with that device current (fg_noalloc.hip).
+void fg_noalloc_setup(int device);
 // fg_stack_destroy for a stack whose work is known complete: does not wait on its stream (which
 // may be shared with, and busy running, another stack).
 void fg_stack_destroy_idle(fg_stack* stack);

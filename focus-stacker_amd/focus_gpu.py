@@ -137,6 +137,7 @@ def lib() -> C.CDLL:
     L.fg_stack_set_profiling.argtypes = [vp, i32]
     L.fg_stack_phase_times.argtypes = [vp, vp, vp]
     L.fg_fuse_stack.argtypes = [vp, u32, vp, vp, vp, C.POINTER(Rect), C.POINTER(StackOpts), vp]
+    L.fg_fuse_stack_device_noalloc.argtypes = [vp, u32, vp, vp, vp, C.POINTER(Rect), C.POINTER(StackOpts), vp]
     L.fg_server_create.argtypes = [vp, C.POINTER(ServerOpts), C.POINTER(vp)]
     L.fg_server_destroy.argtypes = [vp]
     L.fg_server_destroy.restype = None
@@ -242,6 +243,22 @@ class Context:
         check(lib().fg_write_aligned_tiff(self.h, C.byref(desc_for(src)), _p(src), C.byref(p),
                                           C.byref(rr) if rr is not None else None, os.fsencode(prefix), index))
         return f"{prefix}_{index:04d}.tif"
+
+    def fuse_stack_device_noalloc(self, d_frames, descs, poses, roi, d_out: int, method: int = METHOD_PYRAMID,
+                                  window: int = 5, sharpness: float = 0.35, max_levels: int = 0):
+        """fg_fuse_stack_device_noalloc: device frames in, device output written, no device allocation."""
+        n = len(d_frames)
+        dd = (ImageDesc * n)(*descs)
+        ptrs = (C.c_void_p * n)(*d_frames)
+        pz = (Pose * n)(*poses) if poses is not None else None
+        rr = rect(roi)
+        w = roi[2] - roi[0] if roi is not None and poses is not None else descs[0].width
+        h = roi[3] - roi[1] if roi is not None and poses is not None else descs[0].height
+        o = StackOpts(method, window, sharpness, n, w, h, descs[0].sample_type, max_levels, -1.0, -1, None)
+        check(lib().fg_fuse_stack_device_noalloc(self.h, n, C.cast(dd, C.c_void_p), C.cast(ptrs, C.c_void_p),
+                                                 C.cast(pz, C.c_void_p) if pz is not None else None,
+                                                 C.byref(rr) if rr is not None else None, C.byref(o),
+                                                 C.c_void_p(d_out)))
 
     def remap_device(self, d_src: int, d: ImageDesc, p: Pose, roi, d_dst: int, stream: int | None = None):
         rr = rect(roi)

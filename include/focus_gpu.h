@@ -213,6 +213,24 @@ int fg_stack_phase_times(fg_stack* stack, double* ms, uint32_t* launches);
 int fg_fuse_stack(fg_context* ctx, uint32_t n, const fg_image_desc* descs, const void* const* pxs,
                   const fg_pose* poses, const fg_rect* roi, const fg_stack_opts* opts, void* out_px);
 
+/* ---- Allocation-free one-shot fusion ----
+ * No counterpart in the reference.  For callers that cannot allocate device memory: code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc.  fg_fuse_stack for frames
+ * already in device memory (d_frames[k], described by descs[k]) into a device buffer d_out of
+ * W*H*3 samples, with every working buffer carved from a fixed scratch area of
+ * FG_NOALLOC_SCRATCH_BYTES that each GPU holds from the library's load until the process ends.  It
+ * calls no hipMalloc or hipFree and creates no stream: the work runs on opts->stream, or the
+ * context's stream when that is NULL, and the call returns when d_out is written.  The frame cache
+ * is used when it fits the area, else the uncached second pass (fed again from d_frames); a stack
+ * whose fixed buffers alone exceed the area (about 2.5 megapixels per frame) fails with
+ * FG_ERR_INVALID_ARG and a message giving the bytes it needs.  Every call on one GPU shares the
+ * area, which keeps the last call's contents: the library runs one such call per GPU at a time,
+ * and work that overlaps a call on the same GPU must not touch the area (nothing outside the
+ * library can address it). */
+#define FG_NOALLOC_SCRATCH_BYTES ((size_t)256 << 20)
+int fg_fuse_stack_device_noalloc(fg_context* ctx, uint32_t n, const fg_image_desc* descs, const void* const* d_frames,
+                                 const fg_pose* poses, const fg_rect* roi, const fg_stack_opts* opts, void* d_out);
+
 /* ---- Serving: continuous batching of whole stacks ----
  * No counterpart in the reference, which fuses one stack per process.  A server keeps up to
  * `slots` stacks in flight, each on its own HIP stream, so the kernels of uneven stacks overlap
