@@ -181,15 +181,21 @@ struct StageBox {
     int ok;          // frame staged
     int x_lo, y_lo;  // first staged source column and row
     int rows, dw;    // staged rows, dwords per staged row
+    int interior;    // every staged dword lies inside the source buffer (no byte-wise assembly)
 };
 
-// Column constants of the fast-path homography for region column gx: B * (x, 0, neg_d).
+// Column constants of the fast-path homography for region column gx: B * (x, 0, neg_d), with the
+// first two rows scaled by the focal length (the remap loop's form; fast_coord keeps the plain one).
 struct ColHomog {
     double cx, cy, cz;
 };
 __device__ __forceinline__ ColHomog col_homog(const RemapParams& p, int gx) {
     const double pxd = (double)(p.roi_left + gx) - p.dcx;
     return ColHomog{p.B[0] * pxd + p.B[2] * p.neg_d, p.B[3] * pxd + p.B[5] * p.neg_d, p.B[6] * pxd + p.B[8] * p.neg_d};
+}
+__device__ __forceinline__ ColHomog col_homog_focal(const RemapParams& p, int gx) {
+    const ColHomog c = col_homog(p, gx);
+    return ColHomog{p.focal * c.cx, p.focal * c.cy, c.cz};
 }
 
 // Fast-path source coordinate of region pixel (gx, gy) from its column constants; false where the
@@ -207,11 +213,13 @@ __device__ __forceinline__ bool fast_coord(const RemapParams& p, const ColHomog&
     return true;
 }
 
-// Bounding box of the four corner coordinates in `cb` (NaN marks an invalid corner).
+// Bounding box of the four corner coordinates in `cb` (NaN marks an invalid corner) in frame f's
+// source.
 template <typename T>
-__device__ __forceinline__ StageBox stage_box(const double* cb, int sw, int sh) {
+__device__ __forceinline__ StageBox stage_box(const double* cb, const FrameDesc& f) {
     using G = StageGeom<T>;
-    StageBox s{0, 0, 0, 0, 0};
+    const int sw = (int)f.sw, sh = (int)f.sh;
+    StageBox s{0, 0, 0, 0, 0, 0};
     double mnx = cb[0], mxx = cb[0], mny = cb[1], mxy = cb[1];
     bool ok = true;
 #pragma unroll
@@ -235,6 +243,13 @@ __device__ __forceinline__ StageBox stage_box(const double* cb, int sw, int sh) 
     s.y_lo = y_lo;
     s.rows = rows;
     s.dw = dw;
+    // the dword-aligned span of the first and last staged rows inside [lead, end of the image)
+    const uint32_t lead = (uint32_t)((uintptr_t)f.src & 3u);
+    const uint32_t total = f.sw * f.sh * 3u * (uint32_t)sizeof(T) + lead;
+    const uint32_t first = (lead + ((uint32_t)y_lo * f.sw + (uint32_t)x_lo) * 3u * (uint32_t)sizeof(T)) & ~3u;
+    const uint32_t last =
+        ((lead + ((uint32_t)(y_lo + rows - 1) * f.sw + (uint32_t)x_lo) * 3u * (uint32_t)sizeof(T)) & ~3u) + 4u * dw;
+    s.interior = first >= lead && last <= total;
     return s;
 }
 
@@ -255,6 +270,26 @@ __device__ __forceinline__ void stage_load(const FrameDesc& f, const StageBox& s
     const __attribute__((address_space(1))) uint32_t* dwp =
         (const __attribute__((address_space(1))) uint32_t*)(base - lead);
     const uint32_t total = f.sw * f.sh * 3u * (uint32_t)sizeof(T) + lead;  // end of the image in the view
+    // block-uniform: plain dword loads.  16-bit frames only: the 8-bit kernel runs at 128 VGPRs, where
+    // this second copy of the loads costs more in register spills than its checks save
+    if (sizeof(T) == 2 && s.interior) {
+#pragma unroll
+        for (int ir = 0; ir < G::NR; ir++) {
+            const int row = wave + 4 * ir;  // wave-uniform
+            if (row >= s.rows) {
+#pragma unroll
+                for (int ic = 0; ic < G::NC; ic++) pf[ir * G::NC + ic] = 0u;
+                continue;
+            }
+            const uint32_t rw = stage_row_byte<T>(f, s, s.y_lo + row, lead) >> 2;
+#pragma unroll
+            for (int ic = 0; ic < G::NC; ic++) {
+                const int col = lane + 64 * ic;
+                pf[ir * G::NC + ic] = col < s.dw ? dwp[rw + (uint32_t)col] : 0u;
+            }
+        }
+        return;
+    }
 #pragma unroll
     for (int ir = 0; ir < G::NR; ir++) {
         const int row = wave + 4 * ir;
@@ -297,9 +332,9 @@ __device__ __forceinline__ void stage_store(const StageBox& s, int lane, int wav
 // A frame's fast-path constants, copied out of its descriptor once per frame (the descriptor sits
 // in global memory, where the kernel's own stores could alias it for the compiler).
 struct FastFrame {
-    double b1, b4, b7;           // the homography's column for the pano y coordinate
+    double b1f, b4f, b7;         // the homography's column for the pano y coordinate (x and y rows times focal)
     double dcy;
-    double focal, csx, csy, scx, scy;
+    double sx0, sy0;             // source centre + centre shift
     int roi_top, legacy;
     uint32_t sw;
     double wlim, hlim;           // interior bounds: x < sw - 1, y < sh - 1
@@ -307,8 +342,8 @@ struct FastFrame {
 
 __device__ __forceinline__ FastFrame fast_frame(const FrameDesc& f) {
     const RemapParams& p = f.p;
-    return FastFrame{p.B[1], p.B[4], p.B[7], p.dcy, p.focal, p.csx, p.csy, p.scx, p.scy, p.roi_top, p.legacy,
-                     f.sw, (double)f.sw - 1.0, (double)f.sh - 1.0};
+    return FastFrame{p.focal * p.B[1], p.focal * p.B[4], p.B[7], p.dcy, p.scx + p.csx, p.scy + p.csy, p.roi_top,
+                     p.legacy, f.sw, (double)f.sw - 1.0, (double)f.sh - 1.0};
 }
 
 // The interior arithmetic of samplePixelBilinear on taps t[12] (rows y0, y0+1 x columns x0, x0+1
@@ -386,14 +421,16 @@ template <typename T, bool STAGED>
 __device__ __forceinline__ bool remap_fast(const FastFrame& ff, const ColHomog& ch, const StageBox& s,
                                            const uint8_t* stage, uint32_t lead, const T* __restrict__ src, int gy,
                                            double band, uint32_t o[3]) {
+    // the reference's c + (f * (l / denom) + shift) with the focal length folded into the homography
+    // rows and fused products: a few ulp apart, which the tie band covers
     const double pyd = (double)(ff.roi_top + gy) - ff.dcy;
-    const double lx = ch.cx + ff.b1 * pyd;
-    const double ly = ch.cy + ff.b4 * pyd;
-    const double denom = -(ch.cz + ff.b7 * pyd);
+    const double lx = fma(ff.b1f, pyd, ch.cx);
+    const double ly = fma(ff.b4f, pyd, ch.cy);
+    const double denom = -fma(ff.b7, pyd, ch.cz);
     if (!(ff.legacy ? denom > 1e-6 : fabs(denom) > 1e-6)) return false;
     const double inv = rcp_f64(denom);
-    const double x = ff.scx + (ff.focal * (lx * inv) + ff.csx);
-    const double y = ff.scy + (ff.focal * (ly * inv) + ff.csy);
+    const double x = fma(lx, inv, ff.sx0);
+    const double y = fma(ly, inv, ff.sy0);
     if (!(x >= 0.0 && y >= 0.0 && x < ff.wlim && y < ff.hlim)) return false;
     const double fx0 = floor(x), fy0 = floor(y);
     const int x0 = (int)fx0, y0 = (int)fy0;
@@ -581,7 +618,7 @@ __global__ void __launch_bounds__(256, sizeof(T) == 1 ? 4 : 3) k_pass1(Pass1Args
     uint32_t pf[G::NR * G::NC];
     corner(a.k0);
     lds_barrier();
-    StageBox box = stage_box<T>(corners, (int)a.frames[a.k0].sw, (int)a.frames[a.k0].sh);
+    StageBox box = stage_box<T>(corners, a.frames[a.k0]);
     if (box.ok) {
         stage_load<T>(a.frames[a.k0], box, lane, wave, pf);
         stage_store<T>(box, lane, wave, pf, stage);
@@ -598,7 +635,7 @@ __global__ void __launch_bounds__(256, sizeof(T) == 1 ? 4 : 3) k_pass1(Pass1Args
         const FastFrame ff = fast_frame(f);
         const T* fsrc = (const T*)f.src;
         const FrameDesc* fptr = a.frames + k;
-        const ColHomog chm = fast ? col_homog(f.p, gx) : ColHomog{0.0, 0.0, 0.0};
+        const ColHomog chm = fast ? col_homog_focal(f.p, gx) : ColHomog{0.0, 0.0, 0.0};
         // remap the region (remapRigidImage); gray (fillAverageFromLoaded) where alpha > 0.  The
         // loop is compiled once per tap source (staged box or global gather), box.ok being the
         // same for the whole block.
@@ -653,8 +690,7 @@ __global__ void __launch_bounds__(256, sizeof(T) == 1 ? 4 : 3) k_pass1(Pass1Args
         lds_barrier();
         const bool full = contrast_ok && sFull[0] && sFull[1] && sFull[2] && sFull[3];
         // the next frame's source box, in flight until the end of this frame
-        StageBox nbox = k + 1 < a.k1 ? stage_box<T>(corners, (int)a.frames[k + 1].sw, (int)a.frames[k + 1].sh)
-                                     : StageBox{0, 0, 0, 0, 0};
+        StageBox nbox = k + 1 < a.k1 ? stage_box<T>(corners, a.frames[k + 1]) : StageBox{0, 0, 0, 0, 0, 0};
         if (nbox.ok) stage_load<T>(a.frames[k + 1], nbox, lane, wave, pf);
         // contrast of this wave's output rows: window-column sums slide down the column in
         // registers (lane = column), the window's columns are added across lanes
