@@ -14,7 +14,7 @@ ROOT = os.path.dirname(HERE)
 LIB = os.path.join(HERE, "libfocus_gpu.so")
 SOURCES = [os.path.join(HERE, "csrc", f) for f in (
     "fg_kernels.hip", "fg_pass.hip", "fg_single.hip", "fg_noalloc.hip", "fg_api.cpp", "fg_server.cpp",
-    "fg_tiff.cpp")]
+    "fg_batch.cpp", "fg_tiff.cpp")]
 FLAGS = [
     "-Wl,--no-undefined",
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared", "-x", "hip",

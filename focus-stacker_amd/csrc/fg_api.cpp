@@ -853,6 +853,46 @@ void fg_stack_destroy_idle(fg_stack* s) {
     free_stack(s, false);
 }
 
+int fg_stack_view(fg_stack* s, fg::StackView* v) {
+    if (!s || !v) return fail(FG_ERR_INVALID_ARG, "null argument");
+    if (!s->cached || s->single || s->opts.method != FG_METHOD_PYRAMID_CONTRAST)
+        return fail(FG_ERR_STATE, "only cached pyramid-contrast stacks run in a batch");
+    memset(v, 0, sizeof *v);
+    fg::Pass1Args& a = v->p1;
+    a.frames = nullptr;
+    a.k0 = 0;
+    a.k1 = (int)s->opts.frame_count;
+    a.slot_base = 0;
+    a.sample_bytes = s->sb;
+    a.w = s->W;
+    a.h = s->H;
+    a.window = s->opts.contrast_window_size;
+    a.rgba = s->frames;
+    a.weights = s->weights;
+    a.norm_sum = s->S;
+    a.union_sup = s->U;
+    v->p2 = pass2_args(s, 0, (int)s->opts.frame_count, 0);
+    v->lt = &s->lt;
+    v->out_dev = s->out_dev;
+    v->S = s->S;
+    v->U = s->U;
+    v->px = (size_t)s->W * s->H;
+    v->sb = s->sb;
+    return FG_OK;
+}
+
+int fg_stack_describe_device_frames(fg_stack* s, uint32_t n, const fg_image_desc* d, const void* const* px,
+                                    const fg_pose* poses, const fg_rect* roi, fg::FrameDesc* out) {
+    if (!s || !d || !px || !out) return fail(FG_ERR_INVALID_ARG, "null argument");
+    if (n != s->opts.frame_count) return fail(FG_ERR_INVALID_ARG, "frame count differs from the stack's");
+    s->have_src = false;
+    for (uint32_t k = 0; k < n; k++) {
+        const int rc = describe_frame(s, &d[k], px[k], true, poses ? &poses[k] : nullptr, roi, &out[k]);
+        if (rc) return rc;
+    }
+    return FG_OK;
+}
+
 float fg_context_memory_fraction(const fg_context* ctx) { return ctx ? ctx->memory_fraction : 0.0f; }
 void* fg_context_stream(const fg_context* ctx) { return ctx ? (void*)ctx->stream : nullptr; }
 

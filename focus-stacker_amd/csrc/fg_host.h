@@ -25,6 +25,16 @@ int fg_context_device(const fg_context* ctx);
 float fg_context_memory_fraction(const fg_context* ctx);
 // The context's own stream (hipStream_t).
 void* fg_context_stream(const fg_context* ctx);
+namespace fg {
+struct FrameDesc;
+struct StackView;
+}
+// The buffers and pass arguments of a cached pyramid-contrast stack (FG_ERR_STATE otherwise).
+int fg_stack_view(fg_stack* stack, fg::StackView* view);
+// Validates frames 0..n-1 of a stack (device pointers) as fg_stack_add_frame_device does and writes
+// their pass-1 descriptors to `out`, without running anything.
+int fg_stack_describe_device_frames(fg_stack* stack, uint32_t n, const fg_image_desc* descs, const void* const* frames,
+                                    const fg_pose* poses, const fg_rect* roi, fg::FrameDesc* out);
 // Takes the device address of this GPU's scratch area for fg_fuse_stack_device_noalloc, once per
 // device, with that device current (fg_noalloc.hip).
 void fg_noalloc_setup(int device);

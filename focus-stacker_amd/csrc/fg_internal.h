@@ -101,6 +101,101 @@ struct Pass2Args {
     int level_begin, level_end;   // levels [level_begin, level_end) of the table (end 0 = all)
 };
 hipError_t launch_pass2(const Pass2Args& a, hipStream_t s);
+
+// Level l of a stack's table as the pass-2 kernels read it.
+struct LevelArgs {
+    int w, h;          // level l
+    int nw, nh;        // level l + 1 (unused at the top level)
+    int top;           // l == L - 1: no reduce, no expand
+    size_t lev_off;    // level l within a frame's level buffer (levels >= 1)
+    size_t nlev_off;   // level l + 1 within a frame's level buffer
+    size_t acc_off;    // level l within the planar accumulators
+};
+inline LevelArgs level_args(const LevelTable& lt, int l) {
+    const int L = lt.n;
+    const size_t base1 = L > 1 ? lt.lv[1].off : 0;
+    LevelArgs g;
+    g.w = (int)lt.lv[l].w;
+    g.h = (int)lt.lv[l].h;
+    g.top = l == L - 1;
+    g.nw = g.top ? 0 : (int)lt.lv[l + 1].w;
+    g.nh = g.top ? 0 : (int)lt.lv[l + 1].h;
+    g.lev_off = l >= 1 ? lt.lv[l].off - base1 : 0;
+    g.nlev_off = g.top ? 0 : lt.lv[l + 1].off - base1;
+    g.acc_off = 3 * lt.lv[l].off;
+    return g;
+}
+
+// ---- launch shapes (defaults in fg_tuning.h, set by the autotuner) ----
+struct Tuning {
+    int pass2_min_tiles;
+    int batch_wide_level;
+    int batch_pass1_blocks_per_cu;
+};
+Tuning& tuning();
+
+// ---- many stacks in one schedule (fg_batch.cpp) ----
+// One unit of work of a batched launch: a job (a stack, or a stack at one level) and a tile of it.
+struct BatchItem {
+    uint32_t job;
+    uint16_t bx, by;
+};
+// A stack at one pyramid level: what k_pass2 takes as its two arguments.
+struct Pass2Job {
+    Pass2Args a;
+    LevelArgs g;
+};
+// The tiles of pass 1 over a w x h stack (the grid of a plain launch) and its dynamic LDS bytes.
+void pass1_grid(uint32_t w, uint32_t h, uint32_t window, uint32_t* gx, uint32_t* gy);
+size_t pass1_batch_lds(int sample_bytes, uint32_t window);
+// Blocks of a persistent pass-1 launch that are resident at once on the device.
+int pass1_resident_blocks(int sample_bytes, int device);
+// Pass 1 (pyramid-contrast, frames cached) of every stack in `stacks` over the tiles in `items`,
+// pulled by `blocks` persistent blocks through the counter `queue` (zero before the launch).
+hipError_t launch_pass1_batch(int sample_bytes, uint32_t window, const Pass1Args* stacks, const BatchItem* items,
+                              int n_items, int* queue, int blocks, hipStream_t s);
+// One pyramid level of many stacks: item b on block b, tiles of tw x th (64x16 or 16x16).
+hipError_t launch_pass2_batch(int sample_bytes, bool level0, int tw, int th, const Pass2Job* jobs,
+                              const BatchItem* items, int n_items, hipStream_t s);
+// Tiles of the pass-2 kernel of shape tw x th over a w x h level.
+inline void pass2_tiles(int w, int h, int tw, int th, int* gx, int* gy) {
+    *gx = (w + tw - 1) / tw;
+    *gy = (h + th - 1) / th;
+}
+// The collapse of many stacks: parent level += EXPAND(child level), or, with `out`, the last step
+// fused with the quantize.  Item tiles are 64 x 4 pixels of the parent level.
+struct CollapseJob {
+    float* parent;
+    const float* child;
+    int pw, ph, cw, ch;
+    void* out;        // final step: the stack's quantized output, else nullptr
+};
+hipError_t launch_collapse_batch(const CollapseJob* jobs, const BatchItem* items, int n_items, bool final_step,
+                                 int sample_bytes, hipStream_t s);
+inline void collapse_tiles(int w, int h, int* gx, int* gy) {
+    *gx = (w + 63) / 64;
+    *gy = (h + 3) / 4;
+}
+// Zeroes many word ranges in one launch: item (range, chunk) clears words [chunk * 4096, +4096).
+struct ZeroJob {
+    uint32_t* p;
+    size_t words;
+};
+hipError_t launch_zero_batch(const ZeroJob* jobs, const BatchItem* items, int n_items, hipStream_t s);
+
+// What the batch engine reads of a cached stack: its pass arguments (Pass1Args::frames left for the
+// batch's own frame table), level table, output staging and the sums it clears per run.
+struct StackView {
+    Pass1Args p1;
+    Pass2Args p2;
+    const LevelTable* lt;
+    void* out_dev;
+    float* S;
+    uint8_t* U;
+    size_t px;
+    int sb;
+};
+constexpr size_t kZeroChunkWords = 4096;
 size_t pass2_contrib_floats(const LevelTable* lt, uint32_t frames);
 // level-0 gaussian + mask of slot j, encoded as the level buffers are
 hipError_t launch_level0_encode(const Pass2Args& a, int slot, float4* dst, hipStream_t s);

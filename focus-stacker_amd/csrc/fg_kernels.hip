@@ -79,8 +79,8 @@ struct WindowRegs {
     float v[kExIt];
 };
 
-__device__ __forceinline__ WindowRegs load_window(const float* __restrict__ src, int cw, int ch) {
-    const int cx0 = (int)blockIdx.x * (kExW / 2) - 1, ry0 = (int)blockIdx.y * (kExH / 2) - 1;
+__device__ __forceinline__ WindowRegs load_window(const float* __restrict__ src, int cw, int ch, int bx, int by) {
+    const int cx0 = bx * (kExW / 2) - 1, ry0 = by * (kExH / 2) - 1;
     const size_t cn = (size_t)cw * ch;
     WindowRegs w;
 #pragma unroll
@@ -94,37 +94,43 @@ __device__ __forceinline__ WindowRegs load_window(const float* __restrict__ src,
     return w;
 }
 
-__device__ __forceinline__ TileAcc store_window(float* t, const WindowRegs& w) {
+__device__ __forceinline__ TileAcc store_window(float* t, const WindowRegs& w, int bx, int by) {
 #pragma unroll
     for (int it = 0; it < kExIt; it++) {
         const int k = (int)threadIdx.x + it * 256;
         if (k < 3 * kExPlane) t[k] = w.v[it];
     }
-    return TileAcc{t, (int)blockIdx.x * (kExW / 2) - 1, (int)blockIdx.y * (kExH / 2) - 1};
+    return TileAcc{t, bx * (kExW / 2) - 1, by * (kExH / 2) - 1};
 }
 
-__device__ __forceinline__ TileAcc stage_window(float* t, const float* __restrict__ src, int cw, int ch) {
-    return store_window(t, load_window(src, cw, ch));
+__device__ __forceinline__ TileAcc stage_window(float* t, const float* __restrict__ src, int cw, int ch, int bx,
+                                                int by) {
+    return store_window(t, load_window(src, cw, ch, bx, by), bx, by);
 }
 
 static inline dim3 expand_grid(int w, int h) {
     return dim3((unsigned)((w + kExW - 1) / kExW), (unsigned)((h + kExH - 1) / kExH));
 }
 
-__global__ void __launch_bounds__(256) k_collapse(float* __restrict__ parent, int pw, int ph,
-                                                  const float* __restrict__ child, int cw, int ch) {
+__device__ __forceinline__ void collapse_tile(float* __restrict__ parent, int pw, int ph, const float* __restrict__ child,
+                                              int cw, int ch, int bx, int by) {
     __shared__ float win[3 * kExPlane];
-    const int x = blockIdx.x * kExW + (threadIdx.x & (kExW - 1)), y = blockIdx.y * kExH + threadIdx.x / kExW;
+    const int x = bx * kExW + (threadIdx.x & (kExW - 1)), y = by * kExH + threadIdx.x / kExW;
     const bool in = x < pw && y < ph;
     const size_t n = (size_t)pw * ph, i = in ? (size_t)y * pw + x : 0;
     const float p0 = parent[i], p1 = parent[i + n], p2 = parent[i + 2 * n];  // in flight with the window
-    const TileAcc acc = stage_window(win, child, cw, ch);
+    const TileAcc acc = stage_window(win, child, cw, ch, bx, by);
     __syncthreads();
     if (!in) return;
     const f3 e = expand_at(acc, cw, ch, pw, ph, x, y);
     parent[i] = p0 + e.x;
     parent[i + n] = p1 + e.y;
     parent[i + 2 * n] = p2 + e.z;
+}
+
+__global__ void __launch_bounds__(256) k_collapse(float* __restrict__ parent, int pw, int ph,
+                                                  const float* __restrict__ child, int cw, int ch) {
+    collapse_tile(parent, pw, ph, child, cw, ch, (int)blockIdx.x, (int)blockIdx.y);
 }
 
 template <typename T>
@@ -152,15 +158,15 @@ hipError_t launch_quantize(const float* acc0, uint32_t w, uint32_t h, int sb, vo
 // The last collapse step fused with the quantize: level 0 + EXPAND(level 1), then (int)clamp(v + 0.5)
 // (src/fuse/pyramid.zig:223-269), the same f32 operations without storing the collapsed level 0.
 template <typename T>
-__global__ void __launch_bounds__(256) k_collapse_quantize(const float* __restrict__ parent, int pw, int ph,
-                                                           const float* __restrict__ child, int cw, int ch,
-                                                           T* __restrict__ out) {
+__device__ __forceinline__ void collapse_quantize_tile(const float* __restrict__ parent, int pw, int ph,
+                                                       const float* __restrict__ child, int cw, int ch,
+                                                       T* __restrict__ out, int bx, int by) {
     __shared__ float win[3 * kExPlane];
-    const int x = blockIdx.x * kExW + (threadIdx.x & (kExW - 1)), y = blockIdx.y * kExH + threadIdx.x / kExW;
+    const int x = bx * kExW + (threadIdx.x & (kExW - 1)), y = by * kExH + threadIdx.x / kExW;
     const bool in = x < pw && y < ph;
     const size_t n = (size_t)pw * ph, i = in ? (size_t)y * pw + x : 0;
     const float p0 = parent[i], p1 = parent[i + n], p2 = parent[i + 2 * n];  // in flight with the window
-    const TileAcc acc = stage_window(win, child, cw, ch);
+    const TileAcc acc = stage_window(win, child, cw, ch, bx, by);
     __syncthreads();
     if (!in) return;
     const f3 e = expand_at(acc, cw, ch, pw, ph, x, y);
@@ -171,6 +177,55 @@ __global__ void __launch_bounds__(256) k_collapse_quantize(const float* __restri
         q = q < 0.0f ? 0.0f : (q > SampleMax<T>::f ? SampleMax<T>::f : q);
         out[i * 3 + c] = (T)(uint32_t)q;
     }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_collapse_quantize(const float* __restrict__ parent, int pw, int ph,
+                                                           const float* __restrict__ child, int cw, int ch,
+                                                           T* __restrict__ out) {
+    collapse_quantize_tile<T>(parent, pw, ph, child, cw, ch, out, (int)blockIdx.x, (int)blockIdx.y);
+}
+
+// One collapse step of many stacks: block b runs item b (a stack's parent / child levels, a tile).
+__global__ void __launch_bounds__(256) k_collapse_batch(const CollapseJob* __restrict__ jobs,
+                                                        const BatchItem* __restrict__ items) {
+    const BatchItem w = items[blockIdx.x];
+    const CollapseJob j = jobs[w.job];
+    collapse_tile(j.parent, j.pw, j.ph, j.child, j.cw, j.ch, (int)w.bx, (int)w.by);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_collapse_quantize_batch(const CollapseJob* __restrict__ jobs,
+                                                                 const BatchItem* __restrict__ items) {
+    const BatchItem w = items[blockIdx.x];
+    const CollapseJob j = jobs[w.job];
+    collapse_quantize_tile<T>(j.parent, j.pw, j.ph, j.child, j.cw, j.ch, (T*)j.out, (int)w.bx, (int)w.by);
+}
+
+hipError_t launch_collapse_batch(const CollapseJob* jobs, const BatchItem* items, int n_items, bool final_step, int sb,
+                                 hipStream_t s) {
+    if (n_items <= 0) return hipSuccess;
+    if (!final_step)
+        hipLaunchKernelGGL(k_collapse_batch, dim3((unsigned)n_items), dim3(256), 0, s, jobs, items);
+    else if (sb == 1)
+        hipLaunchKernelGGL((k_collapse_quantize_batch<uint8_t>), dim3((unsigned)n_items), dim3(256), 0, s, jobs, items);
+    else
+        hipLaunchKernelGGL((k_collapse_quantize_batch<uint16_t>), dim3((unsigned)n_items), dim3(256), 0, s, jobs, items);
+    return hipGetLastError();
+}
+
+// Zeroes chunk `bx` of range `job` (words [bx * kZeroChunkWords, +kZeroChunkWords) clipped to it).
+__global__ void __launch_bounds__(256) k_zero_batch(const ZeroJob* __restrict__ jobs, const BatchItem* __restrict__ items) {
+    const BatchItem w = items[blockIdx.x];
+    const ZeroJob j = jobs[w.job];
+    const size_t base = (size_t)w.bx * kZeroChunkWords;
+    for (size_t i = base + threadIdx.x; i < base + kZeroChunkWords && i < j.words; i += 256) j.p[i] = 0u;
+}
+
+hipError_t launch_zero_batch(const ZeroJob* jobs, const BatchItem* items, int n_items, hipStream_t s) {
+    if (n_items <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_zero_batch, dim3((unsigned)n_items), dim3(256), 0, s, jobs, items);
+    return hipGetLastError();
 }
 
 // collapses levels L-1 .. 1 in place: level l-1 += EXPAND(level l) for l = L-1 down to 2
@@ -374,10 +429,11 @@ __global__ void __launch_bounds__(256) k_merge_level0(const T* __restrict__ soft
     f3 ls = mk3((float)soft[i * 3], (float)soft[i * 3 + 1], (float)soft[i * 3 + 2]);  // in flight with the windows
     TileAcc as{}, ah{}, ac{};
     if (child) {
-        const WindowRegs rs = load_window(gs1, cw, ch), rc = load_window(col1, cw, ch);
-        if (mixed) ah = stage_window(win[1], gh1, cw, ch);
-        as = store_window(win[0], rs);
-        ac = store_window(win[2], rc);
+        const int bx = (int)blockIdx.x, by = (int)blockIdx.y;
+        const WindowRegs rs = load_window(gs1, cw, ch, bx, by), rc = load_window(col1, cw, ch, bx, by);
+        if (mixed) ah = stage_window(win[1], gh1, cw, ch, bx, by);
+        as = store_window(win[0], rs, bx, by);
+        ac = store_window(win[2], rc, bx, by);
     }
     __syncthreads();
     if (!in) return;

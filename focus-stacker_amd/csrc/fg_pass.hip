@@ -20,6 +20,7 @@
 #include <type_traits>
 
 #include "fg_device.h"
+#include "fg_tuning.h"
 
 namespace fg {
 
@@ -522,7 +523,7 @@ __device__ __forceinline__ V lane_window_sum(V x, int b) {
 // PACK: the u8 hybrid with its frames cached, whose LDS is packed (alpha bytes, union support as
 // row lane masks) to keep four blocks per CU; likewise compiled apart.
 template <typename T, int RH, bool COL, bool PACK>
-__global__ void __launch_bounds__(256, sizeof(T) == 1 ? 4 : 3) k_pass1(Pass1Args a) {
+__device__ __forceinline__ void pass1_tile(const Pass1Args& a, const int bx, const int by) {
     using P = Pack<T>;
     using PT = typename P::type;
     using G = StageGeom<T>;
@@ -553,7 +554,7 @@ __global__ void __launch_bounds__(256, sizeof(T) == 1 ? 4 : 3) k_pass1(Pass1Args
     __shared__ int sFull[4];                          // per wave: every region pixel it remapped has full alpha
 
     const int W = (int)a.w, H = (int)a.h;
-    const int X0 = blockIdx.x * tw, Y0 = blockIdx.y * th;
+    const int X0 = bx * tw, Y0 = by * th;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const bool contrast_ok = W >= (int)a.window && H >= (int)a.window;
     const size_t fpx = (size_t)W * H;
@@ -837,6 +838,35 @@ __global__ void __launch_bounds__(256, sizeof(T) == 1 ? 4 : 3) k_pass1(Pass1Args
 }
 
 template <typename T, int RH, bool COL, bool PACK>
+__global__ void __launch_bounds__(256, sizeof(T) == 1 ? 4 : 3) k_pass1(Pass1Args a) {
+    pass1_tile<T, RH, COL, PACK>(a, (int)blockIdx.x, (int)blockIdx.y);
+}
+
+// Pass 1 of many stacks in one persistent launch: the blocks pull work items (a stack and one of its
+// tiles, every frame of that stack) from a queue in the order of `items` (longest first) until it
+// runs dry; every block reaches the end of the queue and returns.
+template <typename T, int RH, bool COL, bool PACK>
+__global__ void __launch_bounds__(256, sizeof(T) == 1 ? 4 : 3)
+    k_pass1_batch(const Pass1Args* __restrict__ stacks, const BatchItem* __restrict__ items, int n_items,
+                  int* __restrict__ queue) {
+    __shared__ int s_item;
+    for (;;) {
+        if (threadIdx.x == 0) s_item = atomicAdd(queue, 1);
+        __syncthreads();
+        // the same item for the whole block: made wave-uniform so that its arguments load as scalars
+        const int it = __builtin_amdgcn_readfirstlane(s_item);
+        __syncthreads();  // every thread has the item before thread 0 takes the next one
+        if (it >= n_items) return;
+        const int job = __builtin_amdgcn_readfirstlane((int)items[it].job);
+        const int bx = __builtin_amdgcn_readfirstlane((int)items[it].bx);
+        const int by = __builtin_amdgcn_readfirstlane((int)items[it].by);
+        const Pass1Args a = stacks[job];  // one scalar copy per item, as kernel arguments would be
+        pass1_tile<T, RH, COL, PACK>(a, bx, by);
+        __syncthreads();  // the next item reuses the block's LDS
+    }
+}
+
+template <typename T, int RH, bool COL, bool PACK>
 static hipError_t launch_pass1_k(const Pass1Args& a, hipStream_t s) {
     const size_t lds = pass1_lds_bytes((int)a.window, (int)sizeof(T), a.best != nullptr, pass1_slot_winners(a),
                                        a.norm_sum != nullptr, PACK);
@@ -862,6 +892,61 @@ static hipError_t launch_pass1_t(const Pass1Args& a, hipStream_t s) {
     return launch_pass1_k<T, RH, false, false>(a, s);
 }
 
+void pass1_grid(uint32_t w, uint32_t h, uint32_t window, uint32_t* gx, uint32_t* gy) {
+    const int b = (int)window / 2, rh = pass1_rows((int)window);
+    const uint32_t tw = (uint32_t)(P1C - 2 * b), th = (uint32_t)(rh - 2 * b);
+    *gx = (w + tw - 1) / tw;
+    *gy = (h + th - 1) / th;
+}
+
+size_t pass1_batch_lds(int sb, uint32_t window) { return pass1_lds_bytes((int)window, sb, false, true, true, false); }
+
+template <typename T, int RH>
+static int pass1_batch_occupancy(uint32_t window) {
+    int per_cu = 0;
+    const size_t lds = pass1_batch_lds((int)sizeof(T), window);
+    if (lds > 65536)
+        (void)hipFuncSetAttribute((const void*)k_pass1_batch<T, RH, false, false>,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)k_pass1_batch<T, RH, false, false>, 256,
+                                                     lds) != hipSuccess)
+        per_cu = 1;
+    return per_cu > 0 ? per_cu : 1;
+}
+
+int pass1_resident_blocks(int sb, int device) {
+    int cus = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess || cus <= 0) cus = 256;
+    const int per_cu = sb == 1 ? pass1_batch_occupancy<uint8_t, 32>(5) : pass1_batch_occupancy<uint16_t, 32>(5);
+    return per_cu * cus;
+}
+
+template <typename T, int RH>
+static hipError_t launch_pass1_batch_t(uint32_t window, const Pass1Args* stacks, const BatchItem* items, int n_items,
+                                       int* queue, int blocks, hipStream_t s) {
+    const size_t lds = pass1_batch_lds((int)sizeof(T), window);
+    if (lds > 65536) {
+        hipError_t e = hipFuncSetAttribute((const void*)k_pass1_batch<T, RH, false, false>,
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL((k_pass1_batch<T, RH, false, false>), dim3((unsigned)blocks), dim3(256), lds, s, stacks, items,
+                       n_items, queue);
+    return hipGetLastError();
+}
+
+hipError_t launch_pass1_batch(int sb, uint32_t window, const Pass1Args* stacks, const BatchItem* items, int n_items,
+                              int* queue, int blocks, hipStream_t s) {
+    if (n_items <= 0) return hipSuccess;
+    if (window < 3 || window > 31 || (window & 1) == 0 || blocks <= 0) return hipErrorInvalidValue;
+    const bool big = pass1_rows((int)window) == 64;
+    if (sb == 1)
+        return big ? launch_pass1_batch_t<uint8_t, 64>(window, stacks, items, n_items, queue, blocks, s)
+                   : launch_pass1_batch_t<uint8_t, 32>(window, stacks, items, n_items, queue, blocks, s);
+    return big ? launch_pass1_batch_t<uint16_t, 64>(window, stacks, items, n_items, queue, blocks, s)
+               : launch_pass1_batch_t<uint16_t, 32>(window, stacks, items, n_items, queue, blocks, s);
+}
+
 hipError_t launch_pass1(const Pass1Args& a, hipStream_t s) {
     if (a.k1 <= a.k0) return hipSuccess;
     if (a.window < 3 || a.window > 31 || (a.window & 1) == 0) return hipErrorInvalidValue;
@@ -881,15 +966,6 @@ hipError_t launch_pass1(const Pass1Args& a, hipStream_t s) {
 // Unsupported pixels above level 0 have rgb 0, so -(0 + 1) = -1 decodes to 0.
 
 __device__ __forceinline__ float dec_r(float r) { return r < 0.0f ? -r - 1.0f : r; }
-
-struct LevelArgs {
-    int w, h;          // level l
-    int nw, nh;        // level l + 1 (unused at the top level)
-    int top;           // l == L - 1: no reduce, no expand
-    size_t lev_off;    // level l within a frame's level buffer (levels >= 1)
-    size_t nlev_off;   // level l + 1 within a frame's level buffer
-    size_t acc_off;    // level l within the planar accumulators
-};
 
 template <int TW, int TH>
 struct Tile {
@@ -1013,7 +1089,8 @@ __device__ __forceinline__ float div_by_recip(float w, float s, float y) {
 }
 
 template <typename T, bool L0, int TW, int TH, bool SPLIT>
-__global__ void __launch_bounds__(256) k_pass2(Pass2Args a, LevelArgs g) {
+__device__ __forceinline__ void pass2_tile(const Pass2Args& a, const LevelArgs& g, const int bx, const int by,
+                                           const int bz) {
     using TL = Tile<TW, TH>;
     using PT = typename Pack<T>::type;
     constexpr int RW = TL::RW, RH = TL::RH, NW = TL::NW, NH = TL::NH;
@@ -1030,12 +1107,12 @@ __global__ void __launch_bounds__(256) k_pass2(Pass2Args a, LevelArgs g) {
     __shared__ float hR[NH * TW], hG[NH * TW], hB[NH * TW];   // its horizontal expansion per tile column
     __shared__ int sPart[2];                   // per frame parity: some region pixel lacks support
 
-    const int X0 = blockIdx.x * TW, Y0 = blockIdx.y * TH;
+    const int X0 = bx * TW, Y0 = by * TH;
     const int tx = threadIdx.x % TW, ty = threadIdx.x / TW;
     const int n = g.w * g.h;
     float* accp = a.acc + g.acc_off;
     const float m_unsup = L0 ? 1.0f / (float)a.frame_count : 0.0f;
-    const int kb = SPLIT ? a.k0 + (int)blockIdx.z : a.k0;
+    const int kb = SPLIT ? a.k0 + bz : a.k0;
     const int ke = SPLIT ? kb + 1 : a.k1;
     const bool clamp_regime = (g.w > g.h ? g.w : g.h) > 512;  // exact_expand_max_dim, src/fuse/pyramid.zig:8
     const int ox1 = X0 / 2 - 1, oy1 = Y0 / 2 - 1;
@@ -1359,6 +1436,23 @@ __global__ void __launch_bounds__(256) k_pass2(Pass2Args a, LevelArgs g) {
     }
 }
 
+template <typename T, bool L0, int TW, int TH, bool SPLIT>
+__global__ void __launch_bounds__(256) k_pass2(Pass2Args a, LevelArgs g) {
+    pass2_tile<T, L0, TW, TH, SPLIT>(a, g, (int)blockIdx.x, (int)blockIdx.y, (int)blockIdx.z);
+}
+
+// One pyramid level of many stacks in one launch: block b runs item b (a stack's pass-2 arguments at
+// this level and one of its tiles) over every frame of that stack.
+template <typename T, bool L0, int TW, int TH>
+__global__ void __launch_bounds__(256) k_pass2_batch(const Pass2Job* __restrict__ jobs,
+                                                     const BatchItem* __restrict__ items) {
+    const int job = __builtin_amdgcn_readfirstlane((int)items[blockIdx.x].job);
+    const int bx = __builtin_amdgcn_readfirstlane((int)items[blockIdx.x].bx);
+    const int by = __builtin_amdgcn_readfirstlane((int)items[blockIdx.x].by);
+    const Pass2Job j = jobs[job];
+    pass2_tile<T, L0, TW, TH, false>(j.a, j.g, bx, by, 0);
+}
+
 // acc += contribution of each frame, in frame order (from 0.0f when the frames start at frame 0)
 __global__ void __launch_bounds__(256) k_ordered_sum(float* __restrict__ acc, const float* __restrict__ contrib,
                                                      int nk, size_t n3, int fresh) {
@@ -1369,7 +1463,10 @@ __global__ void __launch_bounds__(256) k_ordered_sum(float* __restrict__ acc, co
     acc[i] = v;
 }
 
-constexpr int kMinTiles = 512;  // two tiles per CU before a level loops frames inside its blocks
+Tuning& tuning() {
+    static Tuning t{FG_TUNE_PASS2_MIN_TILES, FG_TUNE_BATCH_WIDE_LEVEL, FG_TUNE_BATCH_PASS1_BLOCKS_PER_CU};
+    return t;
+}
 
 static int tiles(int w, int h, int tw, int th) { return ((w + tw - 1) / tw) * ((h + th - 1) / th); }
 
@@ -1377,7 +1474,8 @@ size_t pass2_contrib_floats(const LevelTable* lt, uint32_t frames) {
     size_t mx = 0;
     for (int l = 0; l < lt->n; l++) {
         const int w = (int)lt->lv[l].w, h = (int)lt->lv[l].h;
-        if (tiles(w, h, 64, 16) >= kMinTiles || tiles(w, h, 16, 16) >= kMinTiles) continue;
+        const int mt = tuning().pass2_min_tiles;
+        if (tiles(w, h, 64, 16) >= mt || tiles(w, h, 16, 16) >= mt) continue;
         const size_t need = (size_t)3 * w * h * frames;
         mx = need > mx ? need : mx;
     }
@@ -1386,10 +1484,11 @@ size_t pass2_contrib_floats(const LevelTable* lt, uint32_t frames) {
 
 template <typename T, bool L0>
 static hipError_t launch_level(const Pass2Args& a, const LevelArgs& g, int nk, hipStream_t s) {
-    if (tiles(g.w, g.h, 64, 16) >= kMinTiles) {
+    const int mt = tuning().pass2_min_tiles;
+    if (tiles(g.w, g.h, 64, 16) >= mt) {
         dim3 grid((g.w + 63) / 64, (g.h + 15) / 16);
         hipLaunchKernelGGL((k_pass2<T, L0, 64, 16, false>), grid, dim3(256), 0, s, a, g);
-    } else if (tiles(g.w, g.h, 16, 16) >= kMinTiles || nk == 1) {
+    } else if (tiles(g.w, g.h, 16, 16) >= mt || nk == 1) {
         dim3 grid((g.w + 15) / 16, (g.h + 15) / 16);
         hipLaunchKernelGGL((k_pass2<T, L0, 16, 16, false>), grid, dim3(256), 0, s, a, g);
     } else {
@@ -1409,18 +1508,9 @@ static hipError_t pass2_t(const Pass2Args& a, hipStream_t s) {
     const int L = lt.n;
     const int nk = a.k1 - a.k0;
     if (nk <= 0) return hipSuccess;
-    const size_t base1 = L > 1 ? lt.lv[1].off : 0;
     const int lend = a.level_end > 0 && a.level_end < L ? a.level_end : L;
     for (int l = a.level_begin; l < lend; l++) {
-        LevelArgs g;
-        g.w = (int)lt.lv[l].w;
-        g.h = (int)lt.lv[l].h;
-        g.top = l == L - 1;
-        g.nw = g.top ? 0 : (int)lt.lv[l + 1].w;
-        g.nh = g.top ? 0 : (int)lt.lv[l + 1].h;
-        g.lev_off = l >= 1 ? lt.lv[l].off - base1 : 0;
-        g.nlev_off = g.top ? 0 : lt.lv[l + 1].off - base1;
-        g.acc_off = 3 * lt.lv[l].off;
+        const LevelArgs g = level_args(lt, l);
         hipError_t e = l == 0 ? launch_level<T, true>(a, g, nk, s) : launch_level<T, false>(a, g, nk, s);
         if (e != hipSuccess) return e;
     }
@@ -1429,6 +1519,27 @@ static hipError_t pass2_t(const Pass2Args& a, hipStream_t s) {
 
 hipError_t launch_pass2(const Pass2Args& a, hipStream_t s) {
     return a.sample_bytes == 1 ? pass2_t<uint8_t>(a, s) : pass2_t<uint16_t>(a, s);
+}
+
+template <typename T, bool L0>
+static hipError_t launch_pass2_batch_t(int tw, const Pass2Job* jobs, const BatchItem* items, int n_items,
+                                       hipStream_t s) {
+    if (tw == 64)
+        hipLaunchKernelGGL((k_pass2_batch<T, L0, 64, 16>), dim3((unsigned)n_items), dim3(256), 0, s, jobs, items);
+    else
+        hipLaunchKernelGGL((k_pass2_batch<T, L0, 16, 16>), dim3((unsigned)n_items), dim3(256), 0, s, jobs, items);
+    return hipGetLastError();
+}
+
+hipError_t launch_pass2_batch(int sb, bool level0, int tw, int th, const Pass2Job* jobs, const BatchItem* items,
+                              int n_items, hipStream_t s) {
+    if (n_items <= 0) return hipSuccess;
+    if (!((tw == 64 && th == 16) || (tw == 16 && th == 16))) return hipErrorInvalidValue;
+    if (sb == 1)
+        return level0 ? launch_pass2_batch_t<uint8_t, true>(tw, jobs, items, n_items, s)
+                      : launch_pass2_batch_t<uint8_t, false>(tw, jobs, items, n_items, s);
+    return level0 ? launch_pass2_batch_t<uint16_t, true>(tw, jobs, items, n_items, s)
+                  : launch_pass2_batch_t<uint16_t, false>(tw, jobs, items, n_items, s);
 }
 
 // level-0 gaussian + mask of one cached frame in the level-buffer encoding (debug readback)

@@ -18,6 +18,7 @@
 #include <unordered_map>
 #include <vector>
 
+#include "fg_arena.h"
 #include "fg_host.h"
 
 namespace {
@@ -52,83 +53,6 @@ struct Pooled {
     uint64_t last_use;
 };
 
-// Device memory of a server: chunks from hipMalloc, added on demand up to the budget and kept
-// until the server ends, so serving itself never calls hipMalloc or hipFree (hipFree waits for
-// the whole device); stacks take 256-byte aligned ranges, first fit, with coalescing frees.
-struct Arena {
-    struct Chunk {
-        char* base;
-        size_t size;
-        std::map<size_t, size_t> free;  // offset -> length
-    };
-    std::vector<Chunk> chunks;
-    std::unordered_map<void*, std::pair<size_t, size_t>> live;  // pointer -> (chunk, length)
-    size_t total = 0, limit = 0;
-    bool overflow = false;  // a request running alone may grow the arena past the limit
-    static constexpr size_t kMinChunk = (size_t)8 << 30;
-
-    void* take(size_t c, size_t bytes) {
-        auto& fl = chunks[c].free;
-        for (auto it = fl.begin(); it != fl.end(); ++it) {
-            if (it->second < bytes) continue;
-            const size_t off = it->first, rem = it->second - bytes;
-            fl.erase(it);
-            if (rem) fl[off + bytes] = rem;
-            void* p = chunks[c].base + off;
-            live[p] = {c, bytes};
-            return p;
-        }
-        return nullptr;
-    }
-    void* alloc(size_t bytes) {
-        bytes = (bytes + 255) & ~(size_t)255;
-        for (size_t c = 0; c < chunks.size(); c++)
-            if (void* p = take(c, bytes)) return p;
-        size_t sz = std::max(bytes, kMinChunk);
-        if (total + sz > limit) sz = bytes;
-        if (total + sz > limit && !overflow) return nullptr;
-        void* base = nullptr;
-        if (hipMalloc(&base, sz) != hipSuccess) {
-            (void)hipGetLastError();
-            return nullptr;
-        }
-        chunks.push_back(Chunk{(char*)base, sz, {}});
-        chunks.back().free[0] = sz;
-        total += sz;
-        return take(chunks.size() - 1, bytes);
-    }
-    void release(void* p) {
-        auto it = live.find(p);
-        if (it == live.end()) return;
-        const size_t c = it->second.first, len = it->second.second;
-        live.erase(it);
-        auto& fl = chunks[c].free;
-        size_t off = (size_t)((char*)p - chunks[c].base), n = len;
-        auto nx = fl.lower_bound(off);
-        if (nx != fl.end() && off + n == nx->first) {  // merge with the following range
-            n += nx->second;
-            nx = fl.erase(nx);
-        }
-        if (nx != fl.begin()) {  // and with the preceding one
-            auto pv = std::prev(nx);
-            if (pv->first + pv->second == off) {
-                pv->second += n;
-                return;
-            }
-        }
-        fl[off] = n;
-    }
-    void destroy() {
-        for (Chunk& c : chunks) (void)hipFree(c.base);
-        chunks.clear();
-        live.clear();
-        total = 0;
-    }
-};
-
-void* arena_alloc(void* u, size_t bytes) { return ((Arena*)u)->alloc(bytes); }
-void arena_free(void* u, void* p) { ((Arena*)u)->release(p); }
-
 bool same_geometry(const fg_stack_opts& a, const fg_stack_opts& b) {
     return a.method == b.method && a.contrast_window_size == b.contrast_window_size &&
            a.hybrid_sharpness == b.hybrid_sharpness && a.frame_count == b.frame_count && a.width == b.width &&
@@ -155,8 +79,8 @@ struct fg_server {
     std::condition_variable cv_work, cv_done;
     std::thread worker;
     bool stop = false;
-    Arena arena;
-    fg_dev_allocator hook{arena_alloc, arena_free, nullptr};
+    fg::Arena arena;
+    fg_dev_allocator hook{fg::arena_alloc, fg::arena_free, nullptr};
 
     void run();
     void drop_idle();

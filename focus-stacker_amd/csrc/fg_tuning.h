@@ -1,0 +1,13 @@
+// Launch-shape defaults of libfocus_gpu.so on MI355X (gfx950).  Rewritten by
+// `python focus-stacker_amd/autotune.py --write` from its measurements on the GPU; the figures it
+// measured are kept beside it in focus-stacker_amd/tuning_mi355x.json.
+#pragma once
+
+// Single stacks: a pyramid level runs 64x16 tiles while it has at least this many of them, then
+// 16x16 tiles, then (fewer 16x16 tiles than this) one frame per block with an ordered sum.
+#define FG_TUNE_PASS2_MIN_TILES 512
+// Batches: a level's launch uses 64x16 tiles when the stacks' mean width at that level is at least
+// this, else 16x16 tiles.
+#define FG_TUNE_BATCH_WIDE_LEVEL 128
+// Batches: persistent pass-1 blocks per CU (0: as many as fit).
+#define FG_TUNE_BATCH_PASS1_BLOCKS_PER_CU 0

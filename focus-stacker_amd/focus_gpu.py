@@ -71,6 +71,15 @@ class Request(C.Structure):
                 ("hybrid_sharpness", C.c_float), ("max_levels", C.c_uint32), ("out", C.c_void_p)]
 
 
+class BatchOpts(C.Structure):
+    _fields_ = [("memory_fraction", C.c_float), ("plan_cache", C.c_int32)]
+
+
+class Tuning(C.Structure):
+    _fields_ = [("pass2_min_tiles", C.c_int32), ("batch_wide_level", C.c_int32),
+                ("batch_pass1_blocks_per_cu", C.c_int32)]
+
+
 class RequestStatus(C.Structure):
     _fields_ = [("state", C.c_int32), ("queue_ms", C.c_double), ("run_ms", C.c_double), ("total_ms", C.c_double)]
 
@@ -143,6 +152,12 @@ def lib() -> C.CDLL:
     L.fg_server_destroy.restype = None
     L.fg_server_submit.argtypes = [vp, C.POINTER(Request), C.POINTER(C.c_uint64)]
     L.fg_server_wait.argtypes = [vp, C.c_uint64, C.c_double, C.POINTER(RequestStatus)]
+    L.fg_batch_create.argtypes = [vp, C.POINTER(BatchOpts), C.POINTER(vp)]
+    L.fg_batch_destroy.argtypes = [vp]
+    L.fg_batch_destroy.restype = None
+    L.fg_batch_fuse.argtypes = [vp, u32, vp, vp]
+    L.fg_get_tuning.argtypes = [C.POINTER(Tuning)]
+    L.fg_set_tuning.argtypes = [C.POINTER(Tuning)]
     L.fg_device_alloc.argtypes = [vp, C.c_size_t]
     L.fg_device_alloc.restype = vp
     L.fg_device_free.argtypes = [vp, vp]
@@ -425,3 +440,60 @@ class Server:
 
     def __exit__(self, *a):
         self.close()
+
+
+def _request(frames_dev, desc, out_dev, poses, roi, method, window, sharpness, max_levels, keep):
+    """An fg_request over device frames; the ctypes arrays it points to are appended to `keep`."""
+    n = len(frames_dev)
+    descs = (ImageDesc * n)(*([desc] * n)) if isinstance(desc, ImageDesc) else (ImageDesc * n)(*desc)
+    ptrs = (C.c_void_p * n)(*frames_dev)
+    pz = (Pose * n)(*poses) if poses is not None else None
+    rr = rect(roi)
+    keep.extend([descs, ptrs, pz, rr])
+    return Request(n, C.cast(descs, C.c_void_p), C.cast(ptrs, C.c_void_p),
+                   C.cast(pz, C.c_void_p) if pz is not None else None,
+                   C.cast(C.pointer(rr), C.c_void_p) if rr is not None else None,
+                   method, window, sharpness, max_levels, out_dev)
+
+
+class Batch:
+    """Batched fusion (fg_batch_*): many requests of device frames fused with one persistent pass-1
+    launch and one launch per pyramid level per group, the group's schedule captured as a HIP graph."""
+
+    def __init__(self, ctx: Context, memory_fraction: float = 0.0, plan_cache: int = 0):
+        self.h_ = C.c_void_p()
+        check(lib().fg_batch_create(ctx.h, C.byref(BatchOpts(memory_fraction, plan_cache)), C.byref(self.h_)))
+
+    def fuse(self, requests) -> list[dict]:
+        """requests: dicts with frames (device pointers), desc (ImageDesc or one per frame), out (device
+        pointer) and optionally poses, roi, method, window, sharpness, max_levels."""
+        keep = []
+        arr = (Request * len(requests))(*[
+            _request(r["frames"], r["desc"], r["out"], r.get("poses"), r.get("roi"), r.get("method", METHOD_PYRAMID),
+                     r.get("window", 5), r.get("sharpness", 0.35), r.get("max_levels", 0), keep) for r in requests])
+        st = (RequestStatus * len(requests))()
+        check(lib().fg_batch_fuse(self.h_, len(requests), C.cast(arr, C.c_void_p), C.cast(st, C.c_void_p)))
+        return [{"state": x.state, "queue_ms": x.queue_ms, "run_ms": x.run_ms, "total_ms": x.total_ms} for x in st]
+
+    def close(self):
+        if self.h_:
+            lib().fg_batch_destroy(self.h_)
+            self.h_ = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+
+def get_tuning() -> dict:
+    t = Tuning()
+    check(lib().fg_get_tuning(C.byref(t)))
+    return {f: getattr(t, f) for f, _ in Tuning._fields_}
+
+
+def set_tuning(**kw) -> None:
+    cur = get_tuning()
+    cur.update(kw)
+    check(lib().fg_set_tuning(C.byref(Tuning(*[int(cur[f]) for f, _ in Tuning._fields_]))))

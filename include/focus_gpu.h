@@ -278,6 +278,39 @@ int fg_server_submit(fg_server* server, const fg_request* req, uint64_t* ticket)
  * releases its ticket; on timeout *st holds QUEUED or RUNNING. */
 int fg_server_wait(fg_server* server, uint64_t ticket, double timeout_ms, fg_request_status* st);
 
+/* ---- Batched fusion: many stacks in one launch schedule ----
+ * No counterpart in the reference, which fuses one stack per process.  fg_batch_fuse takes many
+ * requests (device frames and outputs, as fg_server_submit does) and cuts them, in order, into
+ * groups of pyramid-contrast stacks of one sample type and contrast window whose frame caches fit the
+ * memory budget together.  A group runs as one persistent pass-1 launch over every stack's tiles, one
+ * pass-2 launch per pyramid level and one collapse launch per level, captured as a HIP graph the first
+ * time that group geometry (the ordered list of stack shapes) runs and replayed when it recurs (the
+ * batch keeps up to plan_cache geometries with their buffers).  Other methods, and stacks too large
+ * to cache, run through the plain stack calls.  Every result equals the stack fused alone, and the
+ * call returns when every output is written; status (optional) receives one entry per request,
+ * with run_ms the device time of its group and total_ms from the call's start to its group's end. */
+typedef struct fg_batch fg_batch;
+typedef struct fg_batch_opts {
+    float memory_fraction;     /* share of free HBM the batch may hold (0 = the context's; <= 0 there: all but 1 GB) */
+    int32_t plan_cache;        /* group geometries kept with their graphs (0 = 4, < 0 = none) */
+} fg_batch_opts;
+int fg_batch_create(fg_context* ctx, const fg_batch_opts* opts, fg_batch** out);
+void fg_batch_destroy(fg_batch* batch);
+int fg_batch_fuse(fg_batch* batch, uint32_t count, const fg_request* reqs, fg_request_status* status);
+
+/* ---- Launch shapes ----
+ * Process-wide knobs of the launch shapes, whose defaults the project's autotuner measured on MI355X
+ * (focus-stacker_amd/autotune.py).  Results do not depend on them: every shape computes the same
+ * bits.  fg_set_tuning applies to stacks and batch plans created afterwards. */
+typedef struct fg_tuning {
+    int32_t pass2_min_tiles;           /* single stacks: 64x16 tiles while a level has this many, then 16x16, then
+                                          one frame per block */
+    int32_t batch_wide_level;          /* batches: 64x16 level tiles when the mean level width is at least this */
+    int32_t batch_pass1_blocks_per_cu; /* batches: persistent pass-1 blocks per CU (0 = as many as fit) */
+} fg_tuning;
+int fg_get_tuning(fg_tuning* t);
+int fg_set_tuning(const fg_tuning* t);
+
 #ifdef __cplusplus
 }
 #endif
