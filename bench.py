@@ -95,32 +95,48 @@ def cpu_baseline(frames_np, poses, roi, method=0, max_levels=0):
 
 
 MIX_SIZES = [(640, 480), (1920, 1080), (4000, 3000), (6000, 4000)]
+BIG_SIZE = (7952, 5304)
 
 
-def make_mix(fg, synth, torch, dev, n_stacks, seed=7):
-    """Uneven stacks: frame counts U{4..40}, sizes from MIX_SIZES, poses of the 1024x768 config scaled
-    to the size.  Frames are smooth random images made on the GPU, one pool of 40 per size (each
-    stack takes the first n of its size's pool under its own poses)."""
+def _pool(synth, torch, dev, config_id, stack_id, frames, w, h, **kw):
+    spec = synth.config_spec(config_id, stack_id=stack_id, frames=frames, width=w, height=h, **kw)
+    st = synth.Stack(spec, device=dev)
+    out = [st.frame(k).contiguous() for k in range(frames)]
+    del st
+    torch.cuda.synchronize()
+    return out
+
+
+def make_mix(fg, synth, torch, dev, n_stacks, seed=7, big_frac=0.0, sizes=None):
+    """The uneven stack mix: frame counts U{4..40}, sizes drawn from MIX_SIZES, poses of the 1024x768
+    configuration scaled to the size (each stack its own seeded poses); with big_frac, that share of the
+    stacks are 100-143 frames of 7952x5304.  Frames come from the seeded generator: per size one
+    generated focus stack (40 frames; 143 of the 30-frame workload's generator for the large size),
+    whose first n frames a stack of that size takes as its sources."""
     rng = np.random.default_rng(seed)
-    g = torch.Generator(device=dev)
-    g.manual_seed(seed)
+    sizes = sizes or MIX_SIZES
     pools = {}
-    for w, h in MIX_SIZES:
-        fr = []
-        for _ in range(40):
-            small = torch.randint(0, 256, (h // 16 + 2, w // 16 + 2, 3), device=dev, dtype=torch.uint8, generator=g)
-            big = torch.nn.functional.interpolate(small.permute(2, 0, 1)[None].float(), size=(h, w), mode="bicubic",
-                                                  align_corners=False)
-            fr.append(big[0].permute(1, 2, 0).round().clamp(0, 255).to(torch.uint8).contiguous())
-        pools[(w, h)] = fr
+    for i, (w, h) in enumerate(sizes):
+        pools[(w, h)] = _pool(synth, torch, dev, 0, 900 + i, 40, w, h, shift=3.0 * w / 1024.0)
+        print(f"bench: frame pool {w}x{h} ready", file=sys.stderr, flush=True)
+    if big_frac > 0:
+        pools[BIG_SIZE] = _pool(synth, torch, dev, 1, 0, 143, *BIG_SIZE)
+        print("bench: frame pool 7952x5304 ready", file=sys.stderr, flush=True)
     stacks = []
     for i in range(n_stacks):
-        n = int(rng.integers(4, 41))
-        w, h = MIX_SIZES[int(rng.integers(0, len(MIX_SIZES)))]
-        spec = synth.config_spec(0, stack_id=i, frames=n, width=w, height=h, shift=3.0 * w / 1024.0)
+        big = big_frac > 0 and rng.random() < big_frac
+        if big:
+            n = int(rng.integers(100, 144))
+            w, h = BIG_SIZE
+            spec = synth.config_spec(1, stack_id=i, frames=n)
+        else:
+            n = int(rng.integers(4, 41))
+            w, h = sizes[int(rng.integers(0, len(sizes)))]
+            spec = synth.config_spec(0, stack_id=i, frames=n, width=w, height=h, shift=3.0 * w / 1024.0)
         poses = [fg.pose(p) for p in synth.make_poses(spec)]
         roi = fg.common_overlap_roi([(w, h)] * n, poses)
-        stacks.append(dict(n=n, w=w, h=h, frames=[f.data_ptr() for f in pools[(w, h)][:n]], poses=poses, roi=roi))
+        stacks.append(dict(n=n, w=w, h=h, big=big, frames=[f.data_ptr() for f in pools[(w, h)][:n]], poses=poses,
+                           roi=roi))
     return stacks, pools
 
 
@@ -128,63 +144,96 @@ def _roi_px(s):
     return (s["roi"][2] - s["roi"][0]) * (s["roi"][3] - s["roi"][1])
 
 
+def _plain_stack(fg, ctx, s, out_ptr):
+    """One request through the plain per-stack calls (the drop-in path): create, add, finish, destroy."""
+    rw, rh = s["roi"][2] - s["roi"][0], s["roi"][3] - s["roi"][1]
+    st = fg.Stack(ctx, s["n"], rw, rh, fg.SAMPLE_U8)
+    desc = fg.ImageDesc(s["w"], s["h"], 3, 0, fg.SAMPLE_U8)
+    for f, p in zip(s["frames"], s["poses"]):
+        st.add_frame_device(f, desc, p, s["roi"])
+    if st.needs_pass2_frames():
+        for f, p in zip(s["frames"], s["poses"]):
+            st.pass2_frame_device(f, desc, p, s["roi"])
+    st.finish_device(out_ptr)
+    st.sync()
+    st.close()
+
+
 def run_batch(args):
-    """Uneven stacks: the plain per-stack calls (create, add, finish, destroy; one after another)
-    against the server (all submitted at once, --slots in flight)."""
+    """The 64-stack uneven workload three ways, one after the other on the same requests: the plain
+    per-stack calls (sequential), the server (all submitted at once, --slots in flight) and the batch
+    engine (one persistent pass-1 launch and one launch per level per group, the group schedule
+    replayed from its captured graph after a first, capturing run)."""
     import torch
     import focus_gpu as fg
     import synth
     dev = torch.device("cuda:0")
-    stacks, pools = make_mix(fg, synth, torch, dev, args.stacks)
-    torch.cuda.synchronize()
+    sizes = [tuple(int(v) for v in x.split("x")) for x in args.sizes] if args.sizes else None
+    stacks, pools = make_mix(fg, synth, torch, dev, args.stacks, sizes=sizes)
     ctx = fg.Context(0, memory_fraction=0.8)
     px = sum(s["n"] * _roi_px(s) for s in stacks)
-    outs_a, outs_b = [], []
-    for s in stacks:
-        rw, rh = s["roi"][2] - s["roi"][0], s["roi"][3] - s["roi"][1]
-        outs_a.append(torch.empty((rh, rw, 3), dtype=torch.uint8, device=dev))
-        outs_b.append(torch.zeros((rh, rw, 3), dtype=torch.uint8, device=dev))
-
-    def plain(sub):
-        for s, o in sub:
-            rw, rh = s["roi"][2] - s["roi"][0], s["roi"][3] - s["roi"][1]
-            st = fg.Stack(ctx, s["n"], rw, rh, fg.SAMPLE_U8)
-            desc = fg.ImageDesc(s["w"], s["h"], 3, 0, fg.SAMPLE_U8)
-            for f, p in zip(s["frames"], s["poses"]):
-                st.add_frame_device(f, desc, p, s["roi"])
-            st.finish_device(o.data_ptr())
-            st.sync()
-            st.close()
-
-    print(f"bench: {len(stacks)} stacks, {px / 1e9:.2f} GPix; warmup", file=sys.stderr, flush=True)
-    plain(list(zip(stacks[:4], outs_a[:4])))
-    t0 = time.perf_counter()
-    plain(list(zip(stacks, outs_a)))
-    t_plain = time.perf_counter() - t0
-    print(f"bench: plain {t_plain:.3f} s", file=sys.stderr, flush=True)
-    with fg.Server(ctx, slots=args.slots, memory_fraction=0.8) as srv:
-        def desc_of(s):
-            return fg.ImageDesc(s["w"], s["h"], 3, 0, fg.SAMPLE_U8)
-        for s, o in zip(stacks[:4], outs_b[:4]):
-            srv.wait(srv.submit(s["frames"], desc_of(s), o.data_ptr(), poses=s["poses"], roi=s["roi"]))
-        t0 = time.perf_counter()
-        tickets = [srv.submit(s["frames"], desc_of(s), o.data_ptr(), poses=s["poses"], roi=s["roi"])
-                   for s, o in zip(stacks, outs_b)]
-        status = [srv.wait(t) for t in tickets]
-        t_srv = time.perf_counter() - t0
+    outs = {k: [torch.zeros((s["roi"][3] - s["roi"][1], s["roi"][2] - s["roi"][0], 3), dtype=torch.uint8, device=dev)
+                for s in stacks] for k in ("plain", "server", "batch")}
     torch.cuda.synchronize()
-    same = all(bool(torch.equal(a, b)) for a, b in zip(outs_a, outs_b))
-    lat = [st["total_ms"] for st in status]
-    line = {"metric": "stacks/s, uneven stacks (4-40 frames of 640x480..6000x4000 u8), server vs plain per-stack",
-            "value": round(len(stacks) / t_srv, 3), "unit": "stacks/s", "n_gpus": 1, "steps": 1, "warmup": 1,
-            "ms_per_step": round(t_srv * 1000.0, 3), "higher_is_better": True, "scaling": "weak",
-            "vs_baseline": None, "dtype": "u8", "data": "synthetic",
-            "config": {"workload": f"{len(stacks)} uneven stacks, pyramid-contrast, full depth", "slots": args.slots,
-                       "frame_pixels": px},
-            "plain_stacks_per_s": round(len(stacks) / t_plain, 3), "speedup": round(t_plain / t_srv, 3),
-            "fused_MPix_per_s": {"server": round(px / t_srv / 1e6, 1), "plain": round(px / t_plain / 1e6, 1)},
-            "latency_ms": {"p50": round(pct(lat, 50), 3), "p99": round(pct(lat, 99), 3)},
-            "identical_to_plain": same}
+
+    def reqs(key):
+        return [dict(frames=s["frames"], desc=fg.ImageDesc(s["w"], s["h"], 3, 0, fg.SAMPLE_U8), out=o.data_ptr(),
+                     poses=s["poses"], roi=s["roi"]) for s, o in zip(stacks, outs[key])]
+
+    paths = set(args.paths)
+    t_plain = t_srv = t_first = t_batch = float("nan")
+    st_srv, st_b = [], []
+    print(f"bench: {len(stacks)} stacks, {px / 1e9:.2f} GPix", file=sys.stderr, flush=True)
+    if "plain" in paths:
+        for s, o in list(zip(stacks, outs["plain"]))[:4]:
+            _plain_stack(fg, ctx, s, o.data_ptr())
+        t0 = time.perf_counter()
+        for s, o in zip(stacks, outs["plain"]):
+            _plain_stack(fg, ctx, s, o.data_ptr())
+        t_plain = time.perf_counter() - t0
+        print(f"bench: plain {t_plain:.3f} s", file=sys.stderr, flush=True)
+    if "server" in paths:
+        with fg.Server(ctx, slots=args.slots, memory_fraction=0.8) as srv:
+            for r in reqs("server")[:4]:
+                srv.wait(srv.submit(r["frames"], r["desc"], r["out"], poses=r["poses"], roi=r["roi"]))
+            t0 = time.perf_counter()
+            tickets = [srv.submit(r["frames"], r["desc"], r["out"], poses=r["poses"], roi=r["roi"])
+                       for r in reqs("server")]
+            st_srv = [srv.wait(t) for t in tickets]
+            t_srv = time.perf_counter() - t0
+        print(f"bench: server {t_srv:.3f} s", file=sys.stderr, flush=True)
+    if "batch" in paths:
+        with fg.Batch(ctx, memory_fraction=0.8) as b:
+            t0 = time.perf_counter()
+            b.fuse(reqs("batch"))                       # first run: plans built, schedules captured
+            t_first = time.perf_counter() - t0
+            t0 = time.perf_counter()
+            st_b = b.fuse(reqs("batch"))                # replays the captured schedules
+            t_batch = time.perf_counter() - t0
+        print(f"bench: batch {t_batch:.3f} s (first run {t_first:.3f} s)", file=sys.stderr, flush=True)
+    torch.cuda.synchronize()
+    ran = [k for k in ("plain", "server", "batch") if k in paths]
+    same = all(all(bool(torch.equal(outs[ran[0]][i], outs[k][i])) for k in ran[1:]) for i in range(len(stacks)))
+    lat_b = [x["total_ms"] for x in st_b] or [float("nan")]
+    lat_s = [x["total_ms"] for x in st_srv] or [float("nan")]
+
+    def rnd(v):
+        return None if v != v else round(v, 3)
+    size_txt = ", ".join(f"{w}x{h}" for w, h in (sizes or MIX_SIZES))
+    line = {"metric": f"stacks/s, uneven stacks (4-40 frames of {size_txt} u8): batch vs server vs plain",
+            "value": rnd(len(stacks) / t_batch), "unit": "stacks/s", "n_gpus": 1, "steps": 1, "warmup": 1,
+            "ms_per_step": rnd(t_batch * 1000.0), "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": "fp32", "data": "synthetic",
+            "config": {"workload": f"{len(stacks)} uneven stacks, remap + pyramid-contrast, full depth", "samples": "u8",
+                       "slots": args.slots, "frame_pixels": px},
+            "server_stacks_per_s": rnd(len(stacks) / t_srv), "plain_stacks_per_s": rnd(len(stacks) / t_plain),
+            "batch_first_run_stacks_per_s": rnd(len(stacks) / t_first),
+            "speedup_vs_server": rnd(t_srv / t_batch), "speedup_vs_plain": rnd(t_plain / t_batch),
+            "fused_MPix_per_s": {"batch": rnd(px / t_batch / 1e6), "server": rnd(px / t_srv / 1e6),
+                                 "plain": rnd(px / t_plain / 1e6)},
+            "latency_ms": {"batch_p50": rnd(pct(lat_b, 50)), "batch_p99": rnd(pct(lat_b, 99)),
+                           "server_p50": rnd(pct(lat_s, 50)), "server_p99": rnd(pct(lat_s, 99))},
+            "paths": ran, "identical_outputs": same}
     print(json.dumps(line), flush=True)
 
 
@@ -254,6 +303,10 @@ def main():
     ap.add_argument("--workload", choices=("stack", "batch", "serve"), default="stack",
                     help="stack: one large stack per step (default); batch / serve: uneven stack mixes")
     ap.add_argument("--stacks", type=int, default=64)
+    ap.add_argument("--paths", nargs="+", choices=("plain", "server", "batch"), default=["plain", "server", "batch"],
+                    help="batch workload: which paths to time")
+    ap.add_argument("--sizes", nargs="+", default=None,
+                    help="batch workload: frame sizes WxH of the mix (default 640x480 1920x1080 4000x3000 6000x4000)")
     ap.add_argument("--slots", type=int, default=4)
     ap.add_argument("--requests", type=int, default=48)
     ap.add_argument("--rates", type=float, nargs="+", default=[2.0, 8.0, 32.0])

@@ -17,6 +17,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <deque>
 #include <memory>
 #include <mutex>
 #include <string>
@@ -85,6 +86,7 @@ struct Plan {
     std::vector<int> single_level;         // stacks of one level: quantized directly
     hipGraphExec_t graph = nullptr;
     bool eager = false;                    // capture failed once: launch one by one
+    hipStream_t stream = nullptr;          // the stream of its runs (a plan runs on one stream only)
     uint64_t last_use = 0;
     std::vector<void*> tables;             // arena blocks of the tables
 };
@@ -94,7 +96,7 @@ struct Plan {
 struct fg_batch {
     fg_context* ctx = nullptr;
     int device = 0;
-    hipStream_t stream = nullptr;
+    std::vector<hipStream_t> streams;      // groups alternate over them: one group's pass 2 overlaps the next one's pass 1
     double budget = 0.0;
     int plan_cache = 4;
     fg::Arena arena;
@@ -121,6 +123,7 @@ int upload_table(fg_batch* b, Plan* p, const std::vector<T>& host, DevTable<T>* 
 }
 
 void free_plan(fg_batch* b, Plan* p) {
+    if (p->stream) (void)hipStreamSynchronize(p->stream);
     if (p->staged_pending) (void)hipEventSynchronize(p->staged);
     if (p->graph) (void)hipGraphExecDestroy(p->graph);
     if (p->staged) (void)hipEventDestroy(p->staged);
@@ -130,7 +133,7 @@ void free_plan(fg_batch* b, Plan* p) {
     if (p->queue) b->arena.release(p->queue);
 }
 
-// Plans are only freed once the batch's stream is idle (every run ends synchronised).
+// Frees the least recently used plan (after its stream has finished with it).
 void evict_lru(fg_batch* b) {
     size_t lru = 0;
     for (size_t i = 1; i < b->plans.size(); i++)
@@ -141,15 +144,16 @@ void evict_lru(fg_batch* b) {
 }
 
 // The stacks, buffers and work tables of a group geometry.
-int build_plan(fg_batch* b, const std::vector<Geom>& key, Plan* p) {
+int build_plan(fg_batch* b, const std::vector<Geom>& key, Plan* p, hipStream_t stream) {
     p->key = key;
+    p->stream = stream;
     p->sb = key[0].o.sample_type == FG_SAMPLE_U16 ? 2 : 1;
     p->window = key[0].o.contrast_window_size;
     const fg::Tuning& tu = fg::tuning();
     int rc = FG_OK;
     for (const Geom& g : key) {
         fg_stack_opts o = g.o;
-        o.stream = b->stream;
+        o.stream = stream;
         fg_stack* s = nullptr;
         rc = fg_stack_create_budget(b->ctx, &o, 1e300, &s, &b->hook, false);
         if (rc) return rc;
@@ -282,9 +286,9 @@ int build_plan(fg_batch* b, const std::vector<Geom>& key, Plan* p) {
     return FG_OK;
 }
 
-// The schedule of a plan, enqueued on the batch's stream (eagerly, or under capture).
-int enqueue_schedule(fg_batch* b, Plan* p) {
-    hipStream_t s = b->stream;
+// The schedule of a plan, enqueued on its stream (eagerly, or under capture).
+int enqueue_schedule(Plan* p) {
+    hipStream_t s = p->stream;
     if (fg::launch_zero_batch(p->zjobs.p, p->zitems.p, (int)p->zitems.n, s) != hipSuccess)
         return bfail(FG_ERR_DEVICE, "batch clear launch");
     if (fg::launch_pass1_batch(p->sb, p->window, p->p1.p, p->items1.p, (int)p->items1.n, p->queue, p->blocks1, s) !=
@@ -307,7 +311,8 @@ int enqueue_schedule(fg_batch* b, Plan* p) {
 
 // Runs a plan on the group's requests: frame descriptors up, the schedule (its graph, captured on the
 // plan's first run), the outputs copied to the requests' buffers.
-int run_plan(fg_batch* b, Plan* p, const std::vector<const fg_request*>& reqs) {
+int run_plan(Plan* p, const std::vector<const fg_request*>& reqs) {
+    hipStream_t stream = p->stream;
     if (p->staged_pending) {  // the previous run's upload has to have read the staging
         if (hipEventSynchronize(p->staged) != hipSuccess) return bfail(FG_ERR_DEVICE, "batch staging");
         p->staged_pending = false;
@@ -319,17 +324,17 @@ int run_plan(fg_batch* b, Plan* p, const std::vector<const fg_request*>& reqs) {
         if (rc) return rc;
     }
     if (hipMemcpyAsync(p->frames.p, p->staging, p->frames.n * sizeof(fg::FrameDesc), hipMemcpyHostToDevice,
-                       b->stream) != hipSuccess ||
-        hipEventRecord(p->staged, b->stream) != hipSuccess)
+                       stream) != hipSuccess ||
+        hipEventRecord(p->staged, stream) != hipSuccess)
         return bfail(FG_ERR_DEVICE, "batch frame upload");
     p->staged_pending = true;
     if (!p->graph && !p->eager) {
         hipGraph_t g = nullptr;
         int rc = FG_OK;
-        hipError_t e = hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal);
+        hipError_t e = hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal);
         if (e == hipSuccess) {
-            rc = enqueue_schedule(b, p);
-            e = hipStreamEndCapture(b->stream, &g);
+            rc = enqueue_schedule(p);
+            e = hipStreamEndCapture(stream, &g);
         }
         if (rc == FG_OK && e == hipSuccess && g) e = hipGraphInstantiate(&p->graph, g, nullptr, nullptr, 0);
         if (g) (void)hipGraphDestroy(g);
@@ -341,15 +346,15 @@ int run_plan(fg_batch* b, Plan* p, const std::vector<const fg_request*>& reqs) {
         }
     }
     if (p->graph) {
-        if (hipGraphLaunch(p->graph, b->stream) != hipSuccess) return bfail(FG_ERR_DEVICE, "batch graph launch");
+        if (hipGraphLaunch(p->graph, stream) != hipSuccess) return bfail(FG_ERR_DEVICE, "batch graph launch");
     } else {
-        const int rc = enqueue_schedule(b, p);
+        const int rc = enqueue_schedule(p);
         if (rc) return rc;
     }
     for (size_t i = 0; i < reqs.size(); i++) {
         const fg::StackView& v = p->views[i];
         const size_t bytes = v.px * 3 * (size_t)v.sb;
-        if (hipMemcpyAsync(reqs[i]->out, v.out_dev, bytes, hipMemcpyDeviceToDevice, b->stream) != hipSuccess)
+        if (hipMemcpyAsync(reqs[i]->out, v.out_dev, bytes, hipMemcpyDeviceToDevice, stream) != hipSuccess)
             return bfail(FG_ERR_DEVICE, "batch output copy");
     }
     return FG_OK;
@@ -400,22 +405,22 @@ int validate_request(const fg_request* q, const fg_stack_opts& o) {
     return FG_OK;
 }
 
-// A request that cannot share a batch (another method) runs through the plain stack calls.
-int run_alone(fg_batch* b, const fg_request* q, const fg_stack_opts& o0) {
+// A request that runs as a stack of its own (too large to gain from sharing launches, another method,
+// or too large to cache): the plain stack calls, enqueued on `stream` without waiting.
+int start_single(fg_batch* b, const fg_request* q, const fg_stack_opts& o0, hipStream_t stream, double budget,
+                 fg_stack** out) {
     fg_stack_opts o = o0;
-    o.stream = b->stream;
+    o.stream = stream;
     fg_stack* s = nullptr;
-    const double left = b->budget > b->held ? b->budget - b->held : 0.0;
-    int rc = fg_stack_create_budget(b->ctx, &o, left, &s, &b->hook, false);
+    int rc = fg_stack_create_budget(b->ctx, &o, budget, &s, &b->hook, false);
     if (rc) return rc;
+    *out = s;
     for (uint32_t k = 0; rc == FG_OK && k < q->frame_count; k++)
         rc = fg_stack_add_frame_device(s, &q->descs[k], q->frames[k], q->poses ? &q->poses[k] : nullptr, q->roi);
     if (rc == FG_OK && fg_stack_needs_pass2_frames(s))
         for (uint32_t k = 0; rc == FG_OK && k < q->frame_count; k++)
             rc = fg_stack_pass2_frame_device(s, &q->descs[k], q->frames[k], q->poses ? &q->poses[k] : nullptr, q->roi);
     if (rc == FG_OK) rc = fg_stack_finish_device(s, q->out);
-    (void)hipStreamSynchronize(b->stream);
-    fg_stack_destroy_idle(s);
     return rc;
 }
 
@@ -442,9 +447,15 @@ int fg_batch_create(fg_context* ctx, const fg_batch_opts* opts, fg_batch** out) 
     b->budget = frac > 0.0f ? (double)frac * (double)free_b : (double)free_b - 1.0e9;
     b->arena.limit = b->budget > 0.0 ? (size_t)b->budget : 0;
     b->hook.user = &b->arena;
-    if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) {
-        delete b;
-        return bfail(FG_ERR_DEVICE, "hipStreamCreate");
+    const int ns = std::max(1, std::min(8, fg::tuning().batch_streams));
+    for (int i = 0; i < ns; i++) {
+        hipStream_t st = nullptr;
+        if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) {
+            for (hipStream_t x : b->streams) (void)hipStreamDestroy(x);
+            delete b;
+            return bfail(FG_ERR_DEVICE, "hipStreamCreate");
+        }
+        b->streams.push_back(st);
     }
     *out = b;
     return FG_OK;
@@ -453,11 +464,11 @@ int fg_batch_create(fg_context* ctx, const fg_batch_opts* opts, fg_batch** out) 
 void fg_batch_destroy(fg_batch* b) {
     if (!b) return;
     (void)hipSetDevice(b->device);
-    (void)hipStreamSynchronize(b->stream);
+    for (hipStream_t st : b->streams) (void)hipStreamSynchronize(st);
     for (auto& p : b->plans) free_plan(b, p.get());
     b->plans.clear();
     b->arena.destroy();
-    (void)hipStreamDestroy(b->stream);
+    for (hipStream_t st : b->streams) (void)hipStreamDestroy(st);
     delete b;
 }
 
@@ -466,60 +477,94 @@ int fg_batch_fuse(fg_batch* b, uint32_t count, const fg_request* reqs, fg_reques
     std::lock_guard<std::mutex> lk(b->mu);
     if (hipSetDevice(b->device) != hipSuccess) return bfail(FG_ERR_DEVICE, "hipSetDevice");
     const auto t0 = Clock::now();
+    hipStream_t gstream = b->streams[0];  // the groups' stream; stacks of their own use the others
     hipEvent_t ev0 = nullptr;
-    if (hipEventCreate(&ev0) != hipSuccess || hipEventRecord(ev0, b->stream) != hipSuccess)
+    if (hipEventCreate(&ev0) != hipSuccess || hipEventRecord(ev0, gstream) != hipSuccess)
         return bfail(FG_ERR_DEVICE, "batch events");
-    struct Group {
+    // one entry per unit of work: a group, or a stack of its own
+    struct Track {
         std::vector<uint32_t> idx;
         hipEvent_t start = nullptr, done = nullptr;
         double issued_ms = 0.0;
     };
-    std::vector<Group> groups;
+    std::vector<Track> tracks;
+    auto open_track = [&](hipStream_t st) -> Track& {
+        tracks.emplace_back();
+        Track& t = tracks.back();
+        (void)hipEventCreate(&t.start);
+        (void)hipEventCreate(&t.done);
+        (void)hipEventRecord(t.start, st);
+        t.issued_ms = std::chrono::duration<double, std::milli>(Clock::now() - t0).count();
+        return t;
+    };
     std::vector<int> state(count, FG_REQ_QUEUED);
-    std::vector<std::string> errors(count);
     int first_error = FG_OK;
     std::string first_msg;
     auto note_error = [&](uint32_t i, int rc) {
         state[i] = rc;
-        errors[i] = fg_last_error();
         if (first_error == FG_OK) {
             first_error = rc;
-            first_msg = errors[i];
+            first_msg = fg_last_error();
         }
     };
-    // cut the requests into groups, in order
+    // stacks of their own in flight, oldest first; their buffers go back once their work is done
+    struct Single {
+        fg_stack* s;
+        hipEvent_t done;
+        double bytes;
+    };
+    std::deque<Single> inflight;
+    double inflight_bytes = 0.0;
+    auto retire_oldest = [&]() {
+        Single& x = inflight.front();
+        (void)hipEventSynchronize(x.done);
+        fg_stack_destroy_idle(x.s);
+        (void)hipEventDestroy(x.done);
+        inflight_bytes -= x.bytes;
+        inflight.pop_front();
+    };
+    // room for `need` more bytes: cached plans first (least recently used), then finished singles
+    auto make_room = [&](double need) {
+        while (b->held + inflight_bytes + need > b->budget) {
+            if (!b->plans.empty())
+                evict_lru(b);
+            else if (!inflight.empty())
+                retire_oldest();
+            else
+                break;
+        }
+    };
+    const size_t nsingle = b->streams.size() > 1 ? b->streams.size() - 1 : 1;
+    size_t next_single = 0;
+    const double large_px = (double)fg::tuning().batch_large_kpix * 1000.0;
+    const double group_cap = fg::tuning().batch_group_mib > 0
+                                 ? std::min(b->budget, (double)fg::tuning().batch_group_mib * 1048576.0)
+                                 : b->budget;
+
     std::vector<fg_stack_opts> opts(count);
     std::vector<Geom> cur_key;
     std::vector<uint32_t> cur;
     double cur_bytes = 0.0;
     auto flush = [&]() -> void {
         if (cur.empty()) return;
-        Group g;
-        g.idx = cur;
-        groups.push_back(g);
-        Group& gr = groups.back();
-        // a plan of this geometry, else a new one (least recently used plans make room)
         Plan* p = nullptr;
         for (auto& q : b->plans)
             if (q->key == cur_key) p = q.get();
         int rc = FG_OK;
         if (!p) {
-            // every run ends synchronised, so evicted plans are idle
-            while (!b->plans.empty() &&
-                   ((int)b->plans.size() >= std::max(1, b->plan_cache) || b->held + cur_bytes > b->budget))
-                evict_lru(b);
+            while (!b->plans.empty() && (int)b->plans.size() >= std::max(1, b->plan_cache)) evict_lru(b);
+            make_room(cur_bytes);
             auto np = std::make_unique<Plan>();
-            rc = build_plan(b, cur_key, np.get());
+            rc = build_plan(b, cur_key, np.get(), gstream);
             if (rc == FG_ERR_OOM_DEVICE) {
-                // the arena is full of other plans or fragmented: give every cached plan back and try
-                // again; a group of one stack may then grow the arena past the budget, as a request
-                // served alone does
-                (void)hipStreamSynchronize(b->stream);
+                // the arena is full or fragmented: everything else gives its memory back, and a group of
+                // one stack may then grow the arena past the budget, as a request served alone does
                 free_plan(b, np.get());
                 while (!b->plans.empty()) evict_lru(b);
+                while (!inflight.empty()) retire_oldest();
                 np = std::make_unique<Plan>();
                 b->arena.overflow = cur.size() == 1;
-                rc = build_plan(b, cur_key, np.get());
+                rc = build_plan(b, cur_key, np.get(), gstream);
                 b->arena.overflow = false;
             }
             if (rc == FG_OK) {
@@ -528,22 +573,19 @@ int fg_batch_fuse(fg_batch* b, uint32_t count, const fg_request* reqs, fg_reques
                 b->plans.push_back(std::move(np));
             } else {
                 const std::string msg = fg_last_error();
-                (void)hipStreamSynchronize(b->stream);
                 free_plan(b, np.get());
                 fg_set_error(rc, msg.c_str());
             }
         }
         std::vector<const fg_request*> rq;
         for (uint32_t i : cur) rq.push_back(&reqs[i]);
-        (void)hipEventCreate(&gr.start);
-        (void)hipEventCreate(&gr.done);
-        (void)hipEventRecord(gr.start, b->stream);
-        gr.issued_ms = std::chrono::duration<double, std::milli>(Clock::now() - t0).count();
+        Track& tr = open_track(gstream);
+        tr.idx = cur;
         if (rc == FG_OK) {
             p->last_use = ++b->clock;
-            rc = run_plan(b, p, rq);
+            rc = run_plan(p, rq);
         }
-        (void)hipEventRecord(gr.done, b->stream);
+        (void)hipEventRecord(tr.done, gstream);
         for (uint32_t i : cur) {
             if (rc == FG_OK)
                 state[i] = FG_REQ_DONE;
@@ -551,7 +593,6 @@ int fg_batch_fuse(fg_batch* b, uint32_t count, const fg_request* reqs, fg_reques
                 note_error(i, rc);
         }
         if (b->plan_cache == 0 && p) {  // nothing kept between groups
-            (void)hipStreamSynchronize(b->stream);
             for (size_t k = 0; k < b->plans.size(); k++)
                 if (b->plans[k].get() == p) {
                     b->held -= (double)p->bytes;
@@ -584,56 +625,75 @@ int fg_batch_fuse(fg_batch* b, uint32_t count, const fg_request* reqs, fg_reques
             note_error(i, vrc);
             continue;
         }
-        const bool batchable = opts[i].method == FG_METHOD_PYRAMID_CONTRAST && (double)need <= b->budget;
-        const bool fits = !cur.empty() && opts[i].sample_type == cur_key[0].o.sample_type &&
-                          opts[i].contrast_window_size == cur_key[0].o.contrast_window_size &&
-                          cur_bytes + (double)need <= b->budget;
+        const double roi_px = (double)opts[i].width * (double)opts[i].height;
+        const bool batchable = opts[i].method == FG_METHOD_PYRAMID_CONTRAST && (double)need <= group_cap &&
+                               !(large_px > 0.0 && roi_px >= large_px);
         if (!batchable) {
-            flush();
-            // on its own, through the plain calls: the batch's plans give their memory back first
-            (void)hipStreamSynchronize(b->stream);
-            while (!b->plans.empty()) evict_lru(b);
-            Group g;
-            g.idx = {i};
-            groups.push_back(g);
-            Group& gr = groups.back();
-            (void)hipEventCreate(&gr.start);
-            (void)hipEventCreate(&gr.done);
-            (void)hipEventRecord(gr.start, b->stream);
-            gr.issued_ms = std::chrono::duration<double, std::milli>(Clock::now() - t0).count();
-            const int rc = run_alone(b, q, opts[i]);
-            (void)hipEventRecord(gr.done, b->stream);
+            // a stack of its own on the next single stream, beside the groups (which keep their order)
+            hipStream_t st = b->streams.size() > 1 ? b->streams[1 + next_single++ % nsingle] : gstream;
+            const double cached_need = (double)need;
+            make_room(std::min(cached_need, b->budget));
+            const double left = b->budget - b->held - inflight_bytes;
+            int c2 = 0;
+            const double sneed = (double)fg_plan_stack_bytes(&opts[i], left, &c2, false);
+            Track& tr = open_track(st);
+            tr.idx = {i};
+            fg_stack* s = nullptr;
+            int rc = start_single(b, q, opts[i], st, left, &s);
+            if (rc == FG_ERR_OOM_DEVICE && !s) {  // alone on the device, as the reference runs it: uncached
+                while (!b->plans.empty()) evict_lru(b);
+                while (!inflight.empty()) retire_oldest();
+                b->arena.overflow = true;
+                rc = start_single(b, q, opts[i], st, b->budget, &s);
+                b->arena.overflow = false;
+            }
+            (void)hipEventRecord(tr.done, st);
+            if (s) {
+                hipEvent_t done = nullptr;
+                (void)hipEventCreateWithFlags(&done, hipEventDisableTiming);
+                (void)hipEventRecord(done, st);
+                inflight.push_back(Single{s, done, sneed});
+                inflight_bytes += sneed;
+            }
             if (rc == FG_OK)
                 state[i] = FG_REQ_DONE;
             else
                 note_error(i, rc);
             continue;
         }
+        const bool fits = !cur.empty() && opts[i].sample_type == cur_key[0].o.sample_type &&
+                          opts[i].contrast_window_size == cur_key[0].o.contrast_window_size &&
+                          cur_bytes + (double)need <= group_cap;
         if (!cur.empty() && !fits) flush();
         cur.push_back(i);
         cur_key.push_back(Geom{opts[i]});
         cur_bytes += (double)need;
     }
     flush();
-    const hipError_t e = hipStreamSynchronize(b->stream);
+    hipError_t e = hipSuccess;
+    for (hipStream_t st : b->streams) {
+        const hipError_t e2 = hipStreamSynchronize(st);
+        if (e == hipSuccess) e = e2;
+    }
+    while (!inflight.empty()) retire_oldest();
     const double host_total = std::chrono::duration<double, std::milli>(Clock::now() - t0).count();
     if (status) {
         for (uint32_t i = 0; i < count; i++) status[i] = fg_request_status{state[i], 0.0, 0.0, 0.0};
-        for (const Group& g : groups) {
+        for (const Track& t : tracks) {
             float t_start = 0.0f, t_done = 0.0f;
-            if (g.start && g.done && hipEventElapsedTime(&t_start, ev0, g.start) == hipSuccess &&
-                hipEventElapsedTime(&t_done, ev0, g.done) == hipSuccess) {
-                for (uint32_t i : g.idx) {
-                    status[i].queue_ms = std::max((double)t_start, g.issued_ms);
+            if (t.start && t.done && hipEventElapsedTime(&t_start, ev0, t.start) == hipSuccess &&
+                hipEventElapsedTime(&t_done, ev0, t.done) == hipSuccess) {
+                for (uint32_t i : t.idx) {
+                    status[i].queue_ms = std::max((double)t_start, t.issued_ms);
                     status[i].run_ms = (double)(t_done - t_start);
                     status[i].total_ms = std::min((double)t_done, host_total);
                 }
             }
         }
     }
-    for (Group& g : groups) {
-        if (g.start) (void)hipEventDestroy(g.start);
-        if (g.done) (void)hipEventDestroy(g.done);
+    for (Track& t : tracks) {
+        if (t.start) (void)hipEventDestroy(t.start);
+        if (t.done) (void)hipEventDestroy(t.done);
     }
     (void)hipEventDestroy(ev0);
     if (e != hipSuccess) return bfail(FG_ERR_DEVICE, hipGetErrorString(e));
@@ -647,17 +707,24 @@ int fg_get_tuning(fg_tuning* t) {
     t->pass2_min_tiles = c.pass2_min_tiles;
     t->batch_wide_level = c.batch_wide_level;
     t->batch_pass1_blocks_per_cu = c.batch_pass1_blocks_per_cu;
+    t->batch_group_mib = c.batch_group_mib;
+    t->batch_streams = c.batch_streams;
+    t->batch_large_kpix = c.batch_large_kpix;
     return FG_OK;
 }
 
 int fg_set_tuning(const fg_tuning* t) {
     if (!t) return bfail(FG_ERR_INVALID_ARG, "null argument");
-    if (t->pass2_min_tiles < 1 || t->batch_wide_level < 1 || t->batch_pass1_blocks_per_cu < 0)
+    if (t->pass2_min_tiles < 1 || t->batch_wide_level < 1 || t->batch_pass1_blocks_per_cu < 0 || t->batch_group_mib < 0 ||
+        t->batch_streams < 1 || t->batch_streams > 8 || t->batch_large_kpix < 0)
         return bfail(FG_ERR_INVALID_ARG, "tuning values out of range");
     fg::Tuning& c = fg::tuning();
     c.pass2_min_tiles = t->pass2_min_tiles;
     c.batch_wide_level = t->batch_wide_level;
     c.batch_pass1_blocks_per_cu = t->batch_pass1_blocks_per_cu;
+    c.batch_group_mib = t->batch_group_mib;
+    c.batch_streams = t->batch_streams;
+    c.batch_large_kpix = t->batch_large_kpix;
     return FG_OK;
 }
 

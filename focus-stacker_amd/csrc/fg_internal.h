@@ -131,6 +131,9 @@ struct Tuning {
     int pass2_min_tiles;
     int batch_wide_level;
     int batch_pass1_blocks_per_cu;
+    int batch_group_mib;
+    int batch_streams;
+    int batch_large_kpix;
 };
 Tuning& tuning();
 

@@ -1464,7 +1464,9 @@ __global__ void __launch_bounds__(256) k_ordered_sum(float* __restrict__ acc, co
 }
 
 Tuning& tuning() {
-    static Tuning t{FG_TUNE_PASS2_MIN_TILES, FG_TUNE_BATCH_WIDE_LEVEL, FG_TUNE_BATCH_PASS1_BLOCKS_PER_CU};
+    static Tuning t{FG_TUNE_PASS2_MIN_TILES, FG_TUNE_BATCH_WIDE_LEVEL, FG_TUNE_BATCH_PASS1_BLOCKS_PER_CU,
+                    FG_TUNE_BATCH_GROUP_MIB, FG_TUNE_BATCH_STREAMS,
+                    FG_TUNE_BATCH_LARGE_KPIX};
     return t;
 }
 

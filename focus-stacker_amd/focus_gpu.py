@@ -77,7 +77,8 @@ class BatchOpts(C.Structure):
 
 class Tuning(C.Structure):
     _fields_ = [("pass2_min_tiles", C.c_int32), ("batch_wide_level", C.c_int32),
-                ("batch_pass1_blocks_per_cu", C.c_int32)]
+                ("batch_pass1_blocks_per_cu", C.c_int32), ("batch_group_mib", C.c_int32),
+                ("batch_streams", C.c_int32), ("batch_large_kpix", C.c_int32)]
 
 
 class RequestStatus(C.Structure):

@@ -307,6 +307,11 @@ typedef struct fg_tuning {
                                           one frame per block */
     int32_t batch_wide_level;          /* batches: 64x16 level tiles when the mean level width is at least this */
     int32_t batch_pass1_blocks_per_cu; /* batches: persistent pass-1 blocks per CU (0 = as many as fit) */
+    int32_t batch_group_mib;           /* batches: MiB of stack buffers a group may hold at most (0 = the budget) */
+    int32_t batch_streams;             /* batches: streams of a batch (1..8; read by fg_batch_create): the groups
+                                          run on the first, stacks of their own on the others */
+    int32_t batch_large_kpix;          /* batches: output size (thousands of pixels) from which a pyramid stack
+                                          runs as a stack of its own (0 = never) */
 } fg_tuning;
 int fg_get_tuning(fg_tuning* t);
 int fg_set_tuning(const fg_tuning* t);

@@ -198,3 +198,23 @@ def test_batch_rejects_a_bad_request_and_runs_the_rest(ctx, fg):
             b.fuse([reqs[0], bad, reqs[1]])
     for s, o in zip(stacks, outs):
         assert np.array_equal(_host(o), _alone(fg, ctx, s))
+
+
+@pytest.mark.parametrize("knobs", [dict(batch_large_kpix=0, batch_streams=1),
+                                   dict(batch_large_kpix=1, batch_streams=4, batch_group_mib=64)],
+                         ids=["all-grouped-one-stream", "all-alone-four-streams"])
+def test_batch_outputs_do_not_depend_on_tuning(ctx, fg, knobs):
+    """Every stack in groups on one stream, or every stack on its own over three side streams (with
+    small group caps): the same bits as each stack fused alone."""
+    stacks = _mix(fg, 5, 6, 6)
+    outs = [_out(fg, s) for s in stacks]
+    saved = fg.get_tuning()
+    try:
+        fg.set_tuning(**{**saved, **knobs})
+        with fg.Batch(ctx) as b:
+            status = b.fuse([_request(fg, s, o) for s, o in zip(stacks, outs)])
+    finally:
+        fg.set_tuning(**saved)
+    assert all(st["state"] == fg.REQ_DONE for st in status)
+    for i, (s, o) in enumerate(zip(stacks, outs)):
+        assert np.array_equal(_host(o), _alone(fg, ctx, s)), f"stack {i} differs from the stack alone"
