@@ -113,7 +113,7 @@ __host__ __device__ inline Pass1Lds pass1_lds(int rh, int th, int sb, bool hybri
     o.colour = at; at += colour ? reg * pt : 0;
     o.tK = at;     at += hybrid && cached ? til * 2 : 0;
     o.alpha = at;  at += reg * (pack ? 1 : 2);
-    o.tU = at;     at += acc ? (pack ? (size_t)th * 8 : til) : 0;
+    o.tU = at;     at += acc ? (pack || sb == 2 ? (size_t)th * 8 : til) : 0;
     o.total = (at + 15) & ~(size_t)15;
     return o;
 }
@@ -183,6 +183,7 @@ struct StageBox {
     int x_lo, y_lo;  // first staged source column and row
     int rows, dw;    // staged rows, dwords per staged row
     int interior;    // every staged dword lies inside the source buffer (no byte-wise assembly)
+    int inside;      // every region pixel maps inside the source interior and the staged box
 };
 
 // Column constants of the fast-path homography for region column gx: B * (x, 0, neg_d), with the
@@ -201,12 +202,14 @@ __device__ __forceinline__ ColHomog col_homog_focal(const RemapParams& p, int gx
 
 // Fast-path source coordinate of region pixel (gx, gy) from its column constants; false where the
 // exact path decides.
-__device__ __forceinline__ bool fast_coord(const RemapParams& p, const ColHomog& ch, int gy, double* x, double* y) {
+__device__ __forceinline__ bool fast_coord(const RemapParams& p, const ColHomog& ch, int gy, double* x, double* y,
+                                           double* den = nullptr) {
     const double pyd = (double)(p.roi_top + gy) - p.dcy;
     const double lx = ch.cx + p.B[1] * pyd;
     const double ly = ch.cy + p.B[4] * pyd;
     const double lz = ch.cz + p.B[7] * pyd;
     const double denom = -lz;
+    if (den) *den = denom;
     if (!(p.legacy ? denom > 1e-6 : fabs(denom) > 1e-6)) return false;
     const double inv = rcp_f64(denom);
     *x = p.scx + (p.focal * (lx * inv) + p.csx);
@@ -215,12 +218,19 @@ __device__ __forceinline__ bool fast_coord(const RemapParams& p, const ColHomog&
 }
 
 // Bounding box of the four corner coordinates in `cb` (NaN marks an invalid corner) in frame f's
-// source.
+// source; `cd` holds the homography denominators at the corners.
+//
+// `inside`: the region's footprint lies in the source interior, [m, sw-1-m] x [m, sh-1-m] for a
+// margin m far above the few-ulp difference between the corner formula and the remap loop's.  The
+// denominator is affine in the pixel position, so one sign and a magnitude above the fast path's
+// limit at the four corners hold for the whole region, whose image is then the convex hull of the
+// corner images: every region pixel's taps lie in the box (staged with a one-pixel margin) and in
+// the source, and the remap loop needs no per-pixel checks but the rounding-tie band.
 template <typename T>
-__device__ __forceinline__ StageBox stage_box(const double* cb, const FrameDesc& f) {
+__device__ __forceinline__ StageBox stage_box(const double* cb, const double* cd, const FrameDesc& f) {
     using G = StageGeom<T>;
     const int sw = (int)f.sw, sh = (int)f.sh;
-    StageBox s{0, 0, 0, 0, 0, 0};
+    StageBox s{0, 0, 0, 0, 0, 0, 0};
     double mnx = cb[0], mxx = cb[0], mny = cb[1], mxy = cb[1];
     bool ok = true;
 #pragma unroll
@@ -251,6 +261,10 @@ __device__ __forceinline__ StageBox stage_box(const double* cb, const FrameDesc&
     const uint32_t last =
         ((lead + ((uint32_t)(y_lo + rows - 1) * f.sw + (uint32_t)x_lo) * 3u * (uint32_t)sizeof(T)) & ~3u) + 4u * dw;
     s.interior = first >= lead && last <= total;
+    constexpr double m = 1e-6;
+    const bool same_sign = (cd[0] > 2e-6 && cd[1] > 2e-6 && cd[2] > 2e-6 && cd[3] > 2e-6) ||
+                           (cd[0] < -2e-6 && cd[1] < -2e-6 && cd[2] < -2e-6 && cd[3] < -2e-6);
+    s.inside = same_sign && mnx >= m && mny >= m && mxx <= (double)(sw - 1) - m && mxy <= (double)(sh - 1) - m;
     return s;
 }
 
@@ -418,28 +432,30 @@ __device__ __forceinline__ bool bilinear_taps_lerp64(const uint32_t t[12], doubl
 // staged).  False where the exact path has to decide.  STAGED is the block-uniform choice between
 // the two tap sources: a loop that only reads the staged box holds no global loads, so no
 // iteration waits on the vector-memory counter, which also counts the loop's own RGBA stores.
-template <typename T, bool STAGED>
+template <typename T, bool STAGED, bool INSIDE = false>
 __device__ __forceinline__ bool remap_fast(const FastFrame& ff, const ColHomog& ch, const StageBox& s,
                                            const uint8_t* stage, uint32_t lead, const T* __restrict__ src, int gy,
                                            double band, uint32_t o[3]) {
+    // INSIDE (staged tiles whose whole footprint is in the source interior and the box, StageBox::
+    // inside): the checks below hold for every pixel and are skipped
     // the reference's c + (f * (l / denom) + shift) with the focal length folded into the homography
     // rows and fused products: a few ulp apart, which the tie band covers
     const double pyd = (double)(ff.roi_top + gy) - ff.dcy;
     const double lx = fma(ff.b1f, pyd, ch.cx);
     const double ly = fma(ff.b4f, pyd, ch.cy);
     const double denom = -fma(ff.b7, pyd, ch.cz);
-    if (!(ff.legacy ? denom > 1e-6 : fabs(denom) > 1e-6)) return false;
+    if (!INSIDE && !(ff.legacy ? denom > 1e-6 : fabs(denom) > 1e-6)) return false;
     const double inv = rcp_f64(denom);
     const double x = fma(lx, inv, ff.sx0);
     const double y = fma(ly, inv, ff.sy0);
-    if (!(x >= 0.0 && y >= 0.0 && x < ff.wlim && y < ff.hlim)) return false;
+    if (!INSIDE && !(x >= 0.0 && y >= 0.0 && x < ff.wlim && y < ff.hlim)) return false;
     const double fx0 = floor(x), fy0 = floor(y);
     const int x0 = (int)fx0, y0 = (int)fy0;
     uint32_t t[12];
     if (STAGED) {
-        if (x0 < s.x_lo || y0 < s.y_lo || y0 + 1 >= s.y_lo + s.rows) return false;
+        if (!INSIDE && (x0 < s.x_lo || y0 < s.y_lo || y0 + 1 >= s.y_lo + s.rows)) return false;
         const uint32_t xb = (uint32_t)(x0 - s.x_lo) * 3u * (uint32_t)sizeof(T);
-        if (xb + 6u * (uint32_t)sizeof(T) + 3u > 4u * (uint32_t)s.dw) return false;
+        if (!INSIDE && xb + 6u * (uint32_t)sizeof(T) + 3u > 4u * (uint32_t)s.dw) return false;
         const uint32_t rowb = ff.sw * 3u * (uint32_t)sizeof(T);
         const uint32_t g0 = lead + ((uint32_t)y0 * ff.sw + (uint32_t)s.x_lo) * 3u * (uint32_t)sizeof(T);
         const uint32_t r0 = (uint32_t)(y0 - s.y_lo);
@@ -543,14 +559,16 @@ __device__ __forceinline__ void pass1_tile(const Pass1Args& a, const int bx, con
     PT* colour = (PT*)(smem + L.colour);              // [RH][64] hybrid: packed rgb
     uint32_t* stage = (uint32_t*)(smem + L.stage);    // staged source box of the current frame
     float* tS = (float*)(smem + L.tS);                // [th][64] norm sum of the tile
-    // union support of the tile: a byte per pixel, or for the u8 hybrid (whose winner arrays fill
-    // its LDS) one 64-bit lane mask per tile row, which keeps it to four blocks per CU
-    const bool umask = PACK && acc && hyb;
+    // union support of the tile: one 64-bit lane mask per tile row for 16-bit samples and the u8
+    // hybrid (whose staging / winner arrays fill LDS: the mask keeps them to four blocks per CU),
+    // else a byte per pixel
+    const bool umask = acc && (PACK || sizeof(T) == 2);
     uint8_t* tUb = smem + L.tU;                       // [th][64]
     uint64_t* tUw = (uint64_t*)(smem + L.tU);         // [th]
     float* tB = (float*)(smem + L.tB);                // [th][64] hybrid: best weight
     uint16_t* tK = (uint16_t*)(smem + L.tK);          // [th][64] hybrid, frames cached: its slot
     __shared__ double corners[8];                     // next frame's region corners in its source
+    __shared__ double cden[4];                        // the homography denominators there
     __shared__ int sFull[4];                          // per wave: every region pixel it remapped has full alpha
 
     const int W = (int)a.w, H = (int)a.h;
@@ -572,14 +590,14 @@ __device__ __forceinline__ void pass1_tile(const Pass1Args& a, const int bx, con
     // corner c (threads 0-3) of the region in frame k's source, NaN where not staged
     auto corner = [&](int k) {
         if (threadIdx.x >= 4) return;
-        double x = __builtin_nan(""), y = __builtin_nan("");
+        double x = __builtin_nan(""), y = __builtin_nan(""), d = 0.0;
         if (k < a.k1) {
             const FrameDesc& f = a.frames[k];
             if (f.kind == 0 && f.p.mode == REMAP_HOMOGRAPHY) {
                 const int cx = X0 - b + ((threadIdx.x & 1) ? P1C - 1 : 0);
                 const int cy = Y0 - b + ((threadIdx.x & 2) ? RH - 1 : 0);
                 double sx, sy;
-                if (fast_coord(f.p, col_homog(f.p, cx), cy, &sx, &sy)) {
+                if (fast_coord(f.p, col_homog(f.p, cx), cy, &sx, &sy, &d)) {
                     x = sx;
                     y = sy;
                 }
@@ -587,6 +605,7 @@ __device__ __forceinline__ void pass1_tile(const Pass1Args& a, const int bx, con
         }
         corners[2 * threadIdx.x] = x;
         corners[2 * threadIdx.x + 1] = y;
+        cden[threadIdx.x] = d;
     };
 
 #pragma unroll 1
@@ -619,7 +638,12 @@ __device__ __forceinline__ void pass1_tile(const Pass1Args& a, const int bx, con
     uint32_t pf[G::NR * G::NC];
     corner(a.k0);
     lds_barrier();
-    StageBox box = stage_box<T>(corners, a.frames[a.k0]);
+    // frames are staged where the region lies inside the stack and its footprint inside the source
+    // (StageBox::inside): the staged remap loop then needs no per-pixel checks.  The other tiles
+    // (a border of the stack, a source edge) gather their taps from global memory.
+    const bool region_in = X0 - b >= 0 && Y0 - b >= 0 && X0 - b + P1C <= W && Y0 - b + RH <= H;
+    StageBox box = stage_box<T>(corners, cden, a.frames[a.k0]);
+    box.ok = box.ok && box.inside && region_in;
     if (box.ok) {
         stage_load<T>(a.frames[a.k0], box, lane, wave, pf);
         stage_store<T>(box, lane, wave, pf, stage);
@@ -637,52 +661,71 @@ __device__ __forceinline__ void pass1_tile(const Pass1Args& a, const int bx, con
         const T* fsrc = (const T*)f.src;
         const FrameDesc* fptr = a.frames + k;
         const ColHomog chm = fast ? col_homog_focal(f.p, gx) : ColHomog{0.0, 0.0, 0.0};
-        // remap the region (remapRigidImage); gray (fillAverageFromLoaded) where alpha > 0.  The
-        // loop is compiled once per tap source (staged box or global gather), box.ok being the
-        // same for the whole block.
+        // remap the region (remapRigidImage); gray (fillAverageFromLoaded) where alpha > 0.  One
+        // loop per tap source (staged box or global gather), box.ok being the same for the block.
         // every region pixel inside the stack and at full alpha (a wave vote after the remap): the
         // contrast then needs no support tests and every window holds window^2 samples
         bool lane_full = true;
-        auto remap_rows = [&](auto staged) {
+        // region pixel (r, gy) of this lane as sample o: gray, alpha, hybrid colour, RGBA cache.  IN:
+        // the region lies inside the stack and o came from the fast path (alpha full)
+        auto emit = [&](int r, int gy, const uint32_t o[4], auto in_c) {
+            constexpr bool IN = decltype(in_c)::value;
+            float g = -1.0f;
+            if (IN || o[3] > 0) {
+                const float sc = (float)((o[0] + o[1]) + o[2]);
+                g = div_markstein(sc, SampleMax<T>::gray3, 1.0f / SampleMax<T>::gray3);
+            }
+            const PT c = P::make(o[0], o[1], o[2], o[3]);
+            if (keep_rgba && r >= b && r < RH - b && lane >= b && lane < P1C - b && (IN || (gx < W && gy < H))) {
+                const size_t i = (size_t)gy * W + gx;  // an output pixel of the tile
+                if (sizeof(T) == 1)
+                    ((uint32_t*)rgba)[i] = *(const uint32_t*)&c;
+                else
+                    ((uint2*)rgba)[i] = *(const uint2*)&c;
+            }
+            gray[r * P1C + lane] = g;
+            alph[r * P1C + lane] = (AT)o[3];
+            if (COL) colour[r * P1C + lane] = c;
+            lane_full = lane_full && o[3] == SampleMax<T>::i;
+        };
+        // the fast path where it decides; the rest (pixels within the rounding-tie band, frames the
+        // fast path does not cover) is marked in `redo` and remapped by the exact path after the
+        // loop, the one call site, which keeps the call out of the loops
+        uint32_t redo = 0;
+        if (box.ok) {  // staged: no per-pixel checks
 #pragma unroll 1
-            for (int r = wave; r < RH; r += 4) {
+            for (int r = wave, i = 0; r < RH; r += 4, i++) {
+                const int gy = Y0 - b + r;
+                uint32_t o[4] = {0, 0, 0, SampleMax<T>::i};
+                if (!remap_fast<T, true, true>(ff, chm, box, (const uint8_t*)stage, lead, fsrc, gy, band, o))
+                    redo |= 1u << i;
+                emit(r, gy, o, std::true_type{});
+            }
+        } else {
+#pragma unroll 1
+            for (int r = wave, i = 0; r < RH; r += 4, i++) {
                 const int gy = Y0 - b + r;
                 uint32_t o[4] = {0, 0, 0, 0};
                 if (gx >= 0 && gy >= 0 && gx < W && gy < H) {
-                    if (fast && remap_fast<T, decltype(staged)::value>(ff, chm, box, (const uint8_t*)stage, lead, fsrc,
-                                                                       gy, band, o)) {
+                    if (fast && remap_fast<T, false>(ff, chm, box, (const uint8_t*)stage, lead, fsrc, gy, band, o))
                         o[3] = SampleMax<T>::i;
-                    } else {
-                        const uint64_t e = frame_rgba_exact<T>(*fptr, gx, gy);
-                        o[0] = (uint32_t)e & 0xffffu;
-                        o[1] = (uint32_t)(e >> 16) & 0xffffu;
-                        o[2] = (uint32_t)(e >> 32) & 0xffffu;
-                        o[3] = (uint32_t)(e >> 48);
-                    }
-                }
-                float g = -1.0f;
-                if (o[3] > 0) {
-                    const float sc = (float)((o[0] + o[1]) + o[2]);
-                    g = div_markstein(sc, SampleMax<T>::gray3, 1.0f / SampleMax<T>::gray3);
-                }
-                const PT c = P::make(o[0], o[1], o[2], o[3]);
-                if (keep_rgba && r >= b && r < RH - b && lane >= b && lane < P1C - b && gx < W && gy < H) {
-                    const size_t i = (size_t)gy * W + gx;  // an output pixel of the tile
-                    if (sizeof(T) == 1)
-                        ((uint32_t*)rgba)[i] = *(const uint32_t*)&c;
                     else
-                        ((uint2*)rgba)[i] = *(const uint2*)&c;
+                        redo |= 1u << i;
                 }
-                gray[r * P1C + lane] = g;
-                alph[r * P1C + lane] = (AT)o[3];
-                if (COL) colour[r * P1C + lane] = c;
-                lane_full = lane_full && o[3] == SampleMax<T>::i;
+                emit(r, gy, o, std::false_type{});
             }
-        };
-        if (box.ok)
-            remap_rows(std::true_type{});
-        else
-            remap_rows(std::false_type{});
+        }
+#pragma unroll 1
+        for (int r = wave, i = 0; redo != 0; r += 4, i++) {
+            if (redo & (1u << i)) {
+                redo &= ~(1u << i);
+                const int gy = Y0 - b + r;
+                const uint64_t e = frame_rgba_exact<T>(*fptr, gx, gy);
+                const uint32_t o[4] = {(uint32_t)e & 0xffffu, (uint32_t)(e >> 16) & 0xffffu,
+                                       (uint32_t)(e >> 32) & 0xffffu, (uint32_t)(e >> 48)};
+                emit(r, gy, o, std::false_type{});
+            }
+        }
         {
             const bool wf = __all(lane_full ? 1 : 0);
             if (lane == 0) sFull[wave] = wf ? 1 : 0;
@@ -691,7 +734,8 @@ __device__ __forceinline__ void pass1_tile(const Pass1Args& a, const int bx, con
         lds_barrier();
         const bool full = contrast_ok && sFull[0] && sFull[1] && sFull[2] && sFull[3];
         // the next frame's source box, in flight until the end of this frame
-        StageBox nbox = k + 1 < a.k1 ? stage_box<T>(corners, a.frames[k + 1]) : StageBox{0, 0, 0, 0, 0, 0};
+        StageBox nbox = k + 1 < a.k1 ? stage_box<T>(corners, cden, a.frames[k + 1]) : StageBox{0, 0, 0, 0, 0, 0, 0};
+        nbox.ok = nbox.ok && nbox.inside && region_in;
         if (nbox.ok) stage_load<T>(a.frames[k + 1], nbox, lane, wave, pf);
         // contrast of this wave's output rows: window-column sums slide down the column in
         // registers (lane = column), the window's columns are added across lanes
@@ -746,7 +790,7 @@ __device__ __forceinline__ void pass1_tile(const Pass1Args& a, const int bx, con
                     if (!FULL) n = wsum(vn);
                 }
                 const int y = Y0 + r;
-                if (umask) {  // accumulateBinarySupportMax as one lane-mask OR per row (hybrid layout)
+                if (umask) {  // accumulateBinarySupportMax as one lane-mask OR per row
                     const uint64_t um = __ballot(col_ok && y < H && (FULL || alph[(r + bb) * P1C + lane] > 0));
                     if (lane == 0) atomicOr((unsigned long long*)&tUw[r], (unsigned long long)um);
                 }
@@ -838,7 +882,7 @@ __device__ __forceinline__ void pass1_tile(const Pass1Args& a, const int bx, con
 }
 
 template <typename T, int RH, bool COL, bool PACK>
-__global__ void __launch_bounds__(256, sizeof(T) == 1 ? 4 : 3) k_pass1(Pass1Args a) {
+__global__ void __launch_bounds__(256, 4) k_pass1(Pass1Args a) {
     pass1_tile<T, RH, COL, PACK>(a, (int)blockIdx.x, (int)blockIdx.y);
 }
 
