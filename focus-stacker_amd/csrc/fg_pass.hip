@@ -881,9 +881,29 @@ __device__ __forceinline__ void pass1_tile(const Pass1Args& a, const int bx, con
     }
 }
 
+// Tile of this workgroup in a 2D grid, XCD-aware.  Workgroups are dealt to the 8 XCDs round-robin
+// (linear workgroup id i on XCD i % 8), each XCD with its own L2.  Every XCD takes a contiguous run
+// of tiles in an order that walks the grid in strips of FG_TUNE_XCD_STRIP tile rows, column by
+// column within a strip (1: row-major): the tiles one XCD runs at the same time are neighbours,
+// whose shared halo columns (and the cache lines a tile's row ends only partly use) come into that
+// L2 once.  A bijection of the grid for any strip height.
+__device__ __forceinline__ void xcd_tile(int* bx, int* by) {
+    const int nx = (int)gridDim.x, ny = (int)gridDim.y, n = nx * ny;
+    const int lin = (int)(blockIdx.y * gridDim.x + blockIdx.x);
+    const int q = n >> 3, r = n & 7, x = lin & 7, sl = lin >> 3;
+    const int t = x < r ? x * (q + 1) + sl : r * (q + 1) + (x - r) * q + sl;
+    constexpr int kStrip = FG_TUNE_XCD_STRIP > 1 ? FG_TUNE_XCD_STRIP : 1;
+    const int st = t / (kStrip * nx), w = t - st * (kStrip * nx);
+    const int h = min(kStrip, ny - st * kStrip);
+    *bx = w / h;
+    *by = st * kStrip + w % h;
+}
+
 template <typename T, int RH, bool COL, bool PACK>
 __global__ void __launch_bounds__(256, 4) k_pass1(Pass1Args a) {
-    pass1_tile<T, RH, COL, PACK>(a, (int)blockIdx.x, (int)blockIdx.y);
+    int bx, by;
+    xcd_tile(&bx, &by);
+    pass1_tile<T, RH, COL, PACK>(a, bx, by);
 }
 
 // Pass 1 of many stacks in one persistent launch: the blocks pull work items (a stack and one of its
@@ -1482,7 +1502,13 @@ __device__ __forceinline__ void pass2_tile(const Pass2Args& a, const LevelArgs& 
 
 template <typename T, bool L0, int TW, int TH, bool SPLIT>
 __global__ void __launch_bounds__(256) k_pass2(Pass2Args a, LevelArgs g) {
-    pass2_tile<T, L0, TW, TH, SPLIT>(a, g, (int)blockIdx.x, (int)blockIdx.y, (int)blockIdx.z);
+    if constexpr (SPLIT) {
+        pass2_tile<T, L0, TW, TH, SPLIT>(a, g, (int)blockIdx.x, (int)blockIdx.y, (int)blockIdx.z);
+    } else {
+        int bx, by;
+        xcd_tile(&bx, &by);
+        pass2_tile<T, L0, TW, TH, SPLIT>(a, g, bx, by, 0);
+    }
 }
 
 // One pyramid level of many stacks in one launch: block b runs item b (a stack's pass-2 arguments at

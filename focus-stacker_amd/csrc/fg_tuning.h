@@ -21,3 +21,10 @@
 // own (ordinary launches on the other streams, beside the groups), since it fills the GPU alone
 // (0: every pyramid stack joins a group).
 #define FG_TUNE_BATCH_LARGE_KPIX 4000
+// Single stacks: each XCD runs a contiguous run of pass-1 / pass-2 tiles, ordered in strips of
+// this many tile rows (compile time; 1: row-major).  Measured on the 143 x 7952x5304 u16 stack:
+// row-major runs 117.0 ms, blocks dealt round-robin 120.7 ms, strips of 4-16 rows 119.9-121.2 ms
+// (taller strips slow pass 1).
+#ifndef FG_TUNE_XCD_STRIP
+#define FG_TUNE_XCD_STRIP 1
+#endif
