@@ -107,10 +107,10 @@ def _pool(synth, torch, dev, config_id, stack_id, frames, w, h, **kw):
     return out
 
 
-def make_mix(fg, synth, torch, dev, n_stacks, seed=7, big_frac=0.0, sizes=None):
-    """The uneven stack mix: frame counts U{4..40}, sizes drawn from MIX_SIZES, poses of the 1024x768
-    configuration scaled to the size (each stack its own seeded poses); with big_frac, that share of the
-    stacks are 100-143 frames of 7952x5304.  Frames come from the seeded generator: per size one
+def make_mix(fg, synth, torch, dev, n_stacks, seed=7, sizes=None, n_big=0):
+    """The uneven stack mix: n_stacks stacks of frame counts U{4..40}, sizes drawn from MIX_SIZES,
+    poses of the 1024x768 configuration scaled to the size (each stack its own seeded poses), then
+    n_big stacks of 100-143 frames of 7952x5304.  Frames come from the seeded generator: per size one
     generated focus stack (40 frames; 143 of the 30-frame workload's generator for the large size),
     whose first n frames a stack of that size takes as its sources."""
     rng = np.random.default_rng(seed)
@@ -119,12 +119,12 @@ def make_mix(fg, synth, torch, dev, n_stacks, seed=7, big_frac=0.0, sizes=None):
     for i, (w, h) in enumerate(sizes):
         pools[(w, h)] = _pool(synth, torch, dev, 0, 900 + i, 40, w, h, shift=3.0 * w / 1024.0)
         print(f"bench: frame pool {w}x{h} ready", file=sys.stderr, flush=True)
-    if big_frac > 0:
+    if n_big > 0:
         pools[BIG_SIZE] = _pool(synth, torch, dev, 1, 0, 143, *BIG_SIZE)
         print("bench: frame pool 7952x5304 ready", file=sys.stderr, flush=True)
     stacks = []
-    for i in range(n_stacks):
-        big = big_frac > 0 and rng.random() < big_frac
+    for i in range(n_stacks + n_big):
+        big = i >= n_stacks
         if big:
             n = int(rng.integers(100, 144))
             w, h = BIG_SIZE
@@ -237,53 +237,113 @@ def run_batch(args):
     print(json.dumps(line), flush=True)
 
 
+def _arrivals(rng, rate, duration, min_requests, n_small, n_big, big_share):
+    """Poisson arrival times over `duration` s (at least min_requests of them) and, per request, a
+    template: with probability big_share one of the n_big full-size stacks, else one of the mix."""
+    times, t = [], 0.0
+    while True:
+        t += rng.exponential(1.0 / rate)
+        if t > duration and len(times) >= min_requests:
+            break
+        times.append(t)
+    kinds = [n_small + int(rng.integers(0, n_big)) if n_big and rng.random() < big_share else int(rng.integers(0, n_small))
+             for _ in times]
+    return times, kinds
+
+
 def run_serve(args):
-    """Poisson arrivals of stacks drawn from the uneven mix at several rates; admission by the
-    device-memory budget; per-rate throughput and p50/p99 latency."""
+    """The serving workload: stack requests arrive as a Poisson process at each rate of --rates; each
+    draws its frame count and size from the uneven mix (4-40 frames of 640x480 ... 6000x4000 u8), and
+    5% are 100-143 frames of 7952x5304.  Admission by the device-memory budget (0.8 of free HBM).
+    Each rate offers --duration s of arrivals (at least --min-requests), first to the server (slots
+    in flight, continuous admission), then the same arrival schedule to the plain per-stack path
+    (one request at a time, the drop-in call), whose outputs must equal the server's bit for bit.
+    Latency is submit (arrival) to done; throughput is requests over first arrival to last done."""
     import torch
     import focus_gpu as fg
     import synth
     dev = torch.device("cuda:0")
-    stacks, pools = make_mix(fg, synth, torch, dev, args.stacks)
+    stacks, pools = make_mix(fg, synth, torch, dev, args.stacks, n_big=args.big_templates)
+    n_small = sum(1 for s in stacks if not s["big"])
+    n_big = len(stacks) - n_small
     torch.cuda.synchronize()
     ctx = fg.Context(0, memory_fraction=0.8)
-    outs = {}
-    for i, s in enumerate(stacks):
-        rw, rh = s["roi"][2] - s["roi"][0], s["roi"][3] - s["roi"][1]
-        outs[i] = torch.empty((rh, rw, 3), dtype=torch.uint8, device=dev)
     rng = np.random.default_rng(11)
+
+    def out_for(s):
+        rw, rh = s["roi"][2] - s["roi"][0], s["roi"][3] - s["roi"][1]
+        return torch.empty((rh, rw, 3), dtype=torch.uint8, device=dev)
+
+    def submit(srv, s, out):
+        d = fg.ImageDesc(s["w"], s["h"], 3, 0, fg.SAMPLE_U8)
+        return srv.submit(s["frames"], d, out.data_ptr(), poses=s["poses"], roi=s["roi"])
+
+    def stats(lat, kinds, t_end, n):
+        small = [x for x, k in zip(lat, kinds) if not stacks[k]["big"]] or [float("nan")]
+        big = [x for x, k in zip(lat, kinds) if stacks[k]["big"]] or [float("nan")]
+        return {"stacks_per_s": round(n / t_end, 3), "p50_ms": round(pct(lat, 50), 2), "p99_ms": round(pct(lat, 99), 2),
+                "small_p50_ms": round(pct(small, 50), 2), "small_p99_ms": round(pct(small, 99), 2),
+                "big_p50_ms": round(pct(big, 50), 2) if big[0] == big[0] else None}
+
     rates = []
-    with fg.Server(ctx, slots=args.slots, memory_fraction=0.8) as srv:
-        for rate in args.rates:
-            nreq = args.requests
-            gaps = rng.exponential(1.0 / rate, nreq)
-            pick = rng.integers(0, len(stacks), nreq)
+    same_all = True
+    for rate in args.rates:
+        times, kinds = _arrivals(rng, rate, args.duration, args.min_requests, n_small, n_big, 0.05)
+        outs = [out_for(stacks[k]) for k in kinds]
+        torch.cuda.synchronize()
+        px = sum(stacks[k]["n"] * _roi_px(stacks[k]) for k in kinds)
+        # server (a fresh one per rate, so that the plain path below gets the memory back): submissions
+        # at the arrival times, then the waits
+        with fg.Server(ctx, slots=args.slots, memory_fraction=0.8) as srv:
+            # warm-up: one request of every template (first-use costs: pooled stacks, graph captures)
+            warm = [(submit(srv, s, o), o) for s, o in ((s, out_for(s)) for s in stacks)]
+            for t, _ in warm:
+                srv.wait(t)
+            del warm
             t0 = time.perf_counter()
-            due = t0
             tickets = []
-            for gap, i in zip(gaps, pick):
-                due += gap
-                while time.perf_counter() < due:
-                    time.sleep(min(0.0005, max(0.0, due - time.perf_counter())))
-                s = stacks[int(i)]
-                d = fg.ImageDesc(s["w"], s["h"], 3, 0, fg.SAMPLE_U8)
-                tickets.append(srv.submit(s["frames"], d, outs[int(i)].data_ptr(), poses=s["poses"], roi=s["roi"]))
+            for at, k, o in zip(times, kinds, outs):
+                while time.perf_counter() - t0 < at:
+                    time.sleep(min(0.0005, max(0.0, at - (time.perf_counter() - t0))))
+                tickets.append(submit(srv, stacks[k], o))
             status = [srv.wait(t) for t in tickets]
-            el = time.perf_counter() - t0
-            lat = [st["total_ms"] for st in status]
-            qs = [st["queue_ms"] for st in status]
-            px = sum(stacks[int(i)]["n"] * _roi_px(stacks[int(i)]) for i in pick)
-            r = {"rate": rate, "requests": nreq, "stacks_per_s": round(nreq / el, 3),
-                 "fused_MPix_per_s": round(px / el / 1e6, 1), "p50_ms": round(pct(lat, 50), 3),
-                 "p99_ms": round(pct(lat, 99), 3), "queue_p50_ms": round(pct(qs, 50), 3)}
-            rates.append(r)
-            print(f"bench: serve {r}", file=sys.stderr, flush=True)
-    best = max(rates, key=lambda r: r["stacks_per_s"])
-    line = {"metric": "served stacks/s and p50/p99 latency under Poisson arrivals (uneven stack mix)",
-            "value": best["stacks_per_s"], "unit": "stacks/s", "n_gpus": 1, "steps": len(rates), "warmup": 0,
-            "ms_per_step": None, "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u8",
-            "data": "synthetic", "config": {"workload": f"Poisson arrivals over {len(stacks)} uneven stacks",
-                                            "slots": args.slots, "memory_fraction": 0.8}, "rates": rates}
+        t_srv = max(at + st["total_ms"] / 1000.0 for at, st in zip(times, status))
+        lat_srv = [st["total_ms"] for st in status]
+        srv_line = stats(lat_srv, kinds, t_srv, len(times))
+        print(f"bench: serve rate {rate}/s: {len(times)} requests ({sum(stacks[k]['big'] for k in kinds)} full-size),"
+              f" server {srv_line}", file=sys.stderr, flush=True)
+        # plain path: the same schedule, one request at a time in arrival order
+        outs_p = [out_for(stacks[k]) for k in kinds]
+        torch.cuda.synchronize()
+        lat_p = []
+        t0 = time.perf_counter()
+        for at, k, o in zip(times, kinds, outs_p):
+            while time.perf_counter() - t0 < at:
+                time.sleep(min(0.0005, max(0.0, at - (time.perf_counter() - t0))))
+            _plain_stack(fg, ctx, stacks[k], o.data_ptr())
+            lat_p.append((time.perf_counter() - t0 - at) * 1000.0)
+        t_plain = time.perf_counter() - t0
+        plain_line = stats(lat_p, kinds, t_plain, len(times))
+        same = all(bool(torch.equal(a, b)) for a, b in zip(outs, outs_p))
+        same_all = same_all and same
+        r = {"rate": rate, "requests": len(times), "full_size_requests": int(sum(stacks[k]["big"] for k in kinds)),
+             "offered_s": round(times[-1], 2), "fused_MPix": round(px / 1e6, 1), "server": srv_line,
+             "plain": plain_line, "identical_outputs": same}
+        rates.append(r)
+        print(f"bench: serve rate {rate}/s: plain {plain_line}, identical {same}", file=sys.stderr, flush=True)
+        del outs, outs_p
+        torch.cuda.empty_cache()
+    top = rates[-1]
+    line = {"metric": "served stacks/s with p50/p99 latency under Poisson arrivals "
+                      "(uneven mix 640x480-6000x4000 u8 + 5% 100-143 x 7952x5304), server vs plain path",
+            "value": top["server"]["stacks_per_s"], "unit": "stacks/s", "n_gpus": 1, "steps": len(rates), "warmup": 1,
+            "ms_per_step": None, "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "fp32",
+            "data": "synthetic",
+            "config": {"workload": f"Poisson arrivals at {args.rates} stacks/s, {args.duration} s of arrivals per rate "
+                                   f"(at least {args.min_requests} requests)",
+                       "templates": {"mix": n_small, "full_size": n_big}, "full_size_share": 0.05, "slots": args.slots,
+                       "memory_fraction": 0.8, "samples": "u8"},
+            "rates": rates, "identical_outputs": same_all}
     print(json.dumps(line), flush=True)
 
 
@@ -308,8 +368,12 @@ def main():
     ap.add_argument("--sizes", nargs="+", default=None,
                     help="batch workload: frame sizes WxH of the mix (default 640x480 1920x1080 4000x3000 6000x4000)")
     ap.add_argument("--slots", type=int, default=4)
-    ap.add_argument("--requests", type=int, default=48)
-    ap.add_argument("--rates", type=float, nargs="+", default=[2.0, 8.0, 32.0])
+    ap.add_argument("--rates", type=float, nargs="+", default=[0.5, 2.0, 5.0, 10.0, 20.0],
+                    help="serve workload: Poisson arrival rates (stacks/s)")
+    ap.add_argument("--duration", type=float, default=20.0, help="serve workload: seconds of arrivals per rate")
+    ap.add_argument("--min-requests", type=int, default=30, help="serve workload: requests per rate at least")
+    ap.add_argument("--big-templates", type=int, default=4,
+                    help="serve workload: full-size (100-143 x 7952x5304) stacks the 5%% share draws from")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: runs on one GPU only; a stack would shard across GPUs by frames with an "

@@ -4,6 +4,13 @@
 // and retires it when the slot's completion event fires.  Idle stacks are pooled per slot and
 // reused by requests of the same geometry (fg_stack_reset).  Requests share nothing but the
 // device, so each result equals the request run alone.
+//
+// The scheduler sleeps on a doorbell rung by fg_server_submit and by a host function queued on a
+// slot's stream behind each request's work, so completions are seen as they happen (no polling).
+// The doorbell has a mutex of its own: the host function never takes the server mutex, which the
+// scheduler may hold across HIP calls that wait for the device.  A request's launches are issued
+// with the server mutex released, so submit and wait never queue behind them; the slots, pools,
+// byte counts and arena belong to the scheduler thread alone.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -76,27 +83,45 @@ struct fg_server {
     double in_use = 0.0;                               // bytes of running and pooled stacks
     uint64_t next_id = 1, use_clock = 0;
     std::mutex mu;
-    std::condition_variable cv_work, cv_done;
+    std::condition_variable cv_done;
+    std::mutex door_mu;                                // doorbell: submissions and completions
+    std::condition_variable door_cv;
+    uint64_t rings = 0;
+    bool host_fn = true;                               // completions ring the doorbell
     std::thread worker;
     bool stop = false;
     fg::Arena arena;
     fg_dev_allocator hook{fg::arena_alloc, fg::arena_free, nullptr};
 
     void run();
+    void ring();
     void drop_idle();
     bool admit(std::unique_lock<std::mutex>& lk);
-    void start(int slot, const std::shared_ptr<Request>& r);
+    void start(int slot, const std::shared_ptr<Request>& r, std::unique_lock<std::mutex>& lk);
     void retire(int slot, int state);
 };
+
+void fg_server::ring() {
+    {
+        std::lock_guard<std::mutex> dl(door_mu);
+        rings++;
+    }
+    door_cv.notify_all();
+}
 
 namespace {
 
 int server_fail(int code, const char* msg) { return fg_set_error(code, msg); }
 
+// Runs on the HIP runtime's callback thread when a slot's stream reaches the end of a request: no
+// HIP calls, only the doorbell.
+void ring_cb(void* user) { static_cast<fg_server*>(user)->ring(); }
+
 }  // namespace
 
-// Runs a request's stack calls on its slot's stream (called with the lock held).
-void fg_server::start(int slot, const std::shared_ptr<Request>& r) {
+// Runs a request's stack calls on its slot's stream (called with the lock held; the launches are
+// issued with it released).
+void fg_server::start(int slot, const std::shared_ptr<Request>& r, std::unique_lock<std::mutex>& lk) {
     r->t_start = Clock::now();
     r->state = FG_REQ_RUNNING;
     running[slot] = r;
@@ -143,6 +168,8 @@ void fg_server::start(int slot, const std::shared_ptr<Request>& r) {
         if (rc == FG_OK) in_use += (double)r->bytes;
     }
     r->stack = st;
+    std::string err;
+    lk.unlock();
     const fg_pose* pz = r->has_poses ? r->poses.data() : nullptr;
     const fg_rect* roi = r->has_roi ? &r->roi : nullptr;
     for (uint32_t k = 0; rc == FG_OK && k < r->opts.frame_count; k++)
@@ -152,9 +179,17 @@ void fg_server::start(int slot, const std::shared_ptr<Request>& r) {
             rc = fg_stack_pass2_frame_device(st, &r->descs[k], r->frames[k], pz ? &pz[k] : nullptr, roi);
     if (rc == FG_OK) rc = fg_stack_finish_device(st, r->out);
     if (rc == FG_OK && hipEventRecord(done_ev[slot], streams[slot]) != hipSuccess) rc = FG_ERR_DEVICE;
+    if (rc == FG_OK && host_fn && hipLaunchHostFunc(streams[slot], ring_cb, this) != hipSuccess) {
+        (void)hipGetLastError();
+        host_fn = false;  // the scheduler polls the events instead
+    }
     if (rc != FG_OK) {
-        r->error = fg_last_error();
+        err = fg_last_error();
         (void)hipStreamSynchronize(streams[slot]);
+    }
+    lk.lock();
+    if (rc != FG_OK) {
+        r->error = err;
         retire(slot, rc);
     }
 }
@@ -199,7 +234,7 @@ void fg_server::retire(int slot, int state) {
 
 // Admits queued requests in order while a slot is free and the budget allows (lock held).
 // Returns whether anything was admitted.
-bool fg_server::admit(std::unique_lock<std::mutex>&) {
+bool fg_server::admit(std::unique_lock<std::mutex>& lk) {
     bool any = false;
     while (!queue.empty()) {
         int slot = -1;
@@ -239,7 +274,7 @@ bool fg_server::admit(std::unique_lock<std::mutex>&) {
             r->bytes = need;
         }
         queue.pop_front();
-        start(slot, r);
+        start(slot, r, lk);
         if (r->state == FG_REQ_QUEUED) break;  // requeued: memory comes back as others finish
         any = true;
     }
@@ -248,27 +283,36 @@ bool fg_server::admit(std::unique_lock<std::mutex>&) {
 
 void fg_server::run() {
     (void)hipSetDevice(device);
-    std::unique_lock<std::mutex> lk(mu);
     for (;;) {
-        bool busy = false;
-        for (int i = 0; i < nslots; i++) {
-            if (!running[i]) continue;
-            hipError_t e = hipEventQuery(done_ev[i]);
-            if (e == hipSuccess) {
-                retire(i, FG_REQ_DONE);
-            } else if (e != hipErrorNotReady) {
-                running[i]->error = hipGetErrorString(e);
-                retire(i, FG_ERR_DEVICE);
-            } else {
-                busy = true;
-            }
+        uint64_t seen;
+        {  // rings after this point bring the scheduler round again
+            std::lock_guard<std::mutex> dl(door_mu);
+            seen = rings;
         }
-        if (admit(lk)) busy = true;
-        if (stop && queue.empty() && !busy) break;
-        if (busy)
-            cv_work.wait_for(lk, std::chrono::microseconds(50));
+        bool busy = false, done = false;
+        {
+            std::unique_lock<std::mutex> lk(mu);
+            for (int i = 0; i < nslots; i++) {
+                if (!running[i]) continue;
+                hipError_t e = hipEventQuery(done_ev[i]);
+                if (e == hipSuccess) {
+                    retire(i, FG_REQ_DONE);
+                } else if (e != hipErrorNotReady) {
+                    running[i]->error = hipGetErrorString(e);
+                    retire(i, FG_ERR_DEVICE);
+                } else {
+                    busy = true;
+                }
+            }
+            if (admit(lk)) busy = true;
+            done = stop && queue.empty() && !busy;
+        }
+        if (done) break;
+        std::unique_lock<std::mutex> dl(door_mu);
+        if (busy && !host_fn)
+            door_cv.wait_for(dl, std::chrono::microseconds(50), [&] { return rings != seen; });
         else
-            cv_work.wait(lk, [&] { return stop || !queue.empty(); });
+            door_cv.wait(dl, [&] { return rings != seen; });
     }
 }
 
@@ -317,8 +361,11 @@ void fg_server_destroy(fg_server* s) {
         std::lock_guard<std::mutex> lk(s->mu);
         s->stop = true;
     }
-    s->cv_work.notify_all();
+    s->ring();
     if (s->worker.joinable()) s->worker.join();
+    // the host functions queued behind the last requests may not have run yet: they touch `s`
+    for (hipStream_t st : s->streams)
+        if (st) (void)hipStreamSynchronize(st);
     for (auto& pool : s->idle)
         for (Pooled& p : pool) fg_stack_destroy(p.stack);
     s->arena.destroy();
@@ -364,7 +411,7 @@ int fg_server_submit(fg_server* s, const fg_request* q, uint64_t* ticket) {
         s->by_id[r->id] = r;
         s->queue.push_back(r);
     }
-    s->cv_work.notify_all();
+    s->ring();
     *ticket = r->id;
     return FG_OK;
 }

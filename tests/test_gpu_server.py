@@ -113,3 +113,39 @@ def test_server_rejects_bad_requests(fg, ctx, stacks):
             srv.submit(s["frames"], desc, 1 << 20, poses=s["poses"], roi=s["roi"], window=4)
         with pytest.raises(fg.FgError):
             srv.wait(12345)
+
+
+@pytest.fixture(scope="module")
+def full_size(fg):
+    """A 104-frame stack of 7952x5304 u8 (the 30-frame workload's generator and pose distribution):
+    eight generated frames, taken in turn under 104 seeded poses, so that the frames stay few."""
+    import torch
+    import synth
+    n, distinct = 104, 8
+    spec = synth.config_spec(1, stack_id=41, frames=n)
+    poses = [fg.pose(p) for p in synth.make_poses(spec)]
+    gen = synth.Stack(synth.config_spec(1, stack_id=41, frames=distinct), device=torch.device("cuda:0"))
+    pool = [gen.frame(k).contiguous() for k in range(distinct)]
+    del gen
+    torch.cuda.synchronize()
+    w, h = spec.width, spec.height
+    roi = fg.common_overlap_roi([(w, h)] * n, poses)
+    yield dict(frames=[pool[k % distinct].data_ptr() for k in range(n)], poses=poses, roi=roi, w=w, h=h,
+               method=fg.METHOD_PYRAMID, window=5, pool=pool)
+    del pool
+    torch.cuda.empty_cache()
+
+
+@pytest.mark.parametrize("fraction", [0.8, 0.06], ids=["budget-0.8", "budget-0.06"])
+def test_server_full_size_stack_beside_small_ones(fg, ctx, stacks, full_size, fraction):
+    """A 104-frame 7952x5304 request admitted between small ones: with 0.8 of the free memory it runs
+    cached beside them; with 0.06 (less than its 34 GB frame cache) it is admitted uncached, its
+    second pass remapping the frames again.  Every result equals its stack fused alone."""
+    mix = stacks[:3] + [full_size] + stacks[3:]
+    print("server mix: full-size", full_size["w"], "x", full_size["h"], len(full_size["frames"]), "frames,",
+          "budget", fraction, flush=True)
+    served, status = _served(fg, ctx, mix, slots=4, memory_fraction=fraction)
+    for i, (s, b, st) in enumerate(zip(mix, served, status)):
+        assert st["state"] == fg.REQ_DONE, (i, st)
+        a = _alone(fg, ctx, s)
+        assert np.array_equal(a, b), f"request {i} ({s['w']}x{s['h']}, {len(s['frames'])} frames) differs in the mix"
