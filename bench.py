@@ -50,7 +50,8 @@ def kernel_models(s: int) -> dict:
 # counters on a plain copy of known size (profiles/pmc_traffic_<workload>.md), keyed by
 # (frames, width, height, samples, method, levels)
 TRAFFIC = {
-    (30, 7952, 5304, "u8", "pyramid", 0): {"k_pass1": 20.263e9, "k_pass2_level0": 28.988e9},
+    (143, 7952, 5304, "u16", "pyramid", 0): {"k_pass1": 132.553e9, "k_pass2_level0": 154.338e9},
+    (30, 7952, 5304, "u8", "pyramid", 0): {"k_pass1": 16.248e9, "k_pass2_level0": 20.251e9},
 }
 COPY_PROBE_BYTES = 256 << 20
 
