@@ -481,7 +481,10 @@ def main():
         if launches == 0:
             continue
         avg_ms = total_ms / launches
-        units = n * w * h  # one launch covers every frame of the stack at its level
+        # frame-pixels per launch: the stack's frames over its launches of this kind per step (the
+        # pyramid methods run one pass-1 and one level-0 launch per stack; the single-pass methods
+        # one pass-1 launch per group of frames)
+        units = n * w * h * args.steps // launches
         ach = per_unit * units / (avg_ms / 1000.0) / 1e9
         kern[name] = {"avg_launch_ms": round(avg_ms, 4), "launches": launches, "bytes_per_unit": per_unit,
                       "units_per_launch": units, "achieved": round(ach, 1), "frac": round(ach / PEAK_HBM_GBS, 4)}

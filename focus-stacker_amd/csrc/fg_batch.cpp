@@ -535,7 +535,6 @@ int fg_batch_fuse(fg_batch* b, uint32_t count, const fg_request* reqs, fg_reques
         }
     };
     const size_t nsingle = b->streams.size() > 1 ? b->streams.size() - 1 : 1;
-    size_t next_single = 0;
     const double large_px = (double)fg::tuning().batch_large_kpix * 1000.0;
     const double group_cap = fg::tuning().batch_group_mib > 0
                                  ? std::min(b->budget, (double)fg::tuning().batch_group_mib * 1048576.0)
@@ -605,6 +604,14 @@ int fg_batch_fuse(fg_batch* b, uint32_t count, const fg_request* reqs, fg_reques
         cur_key.clear();
         cur_bytes = 0.0;
     };
+    // classify: stacks of their own, and the batchable ones (grouped in request order)
+    struct SingleReq {
+        uint32_t i;
+        double cost;  // frame-pixels: both passes scale with them
+    };
+    std::vector<SingleReq> singles;
+    std::vector<uint32_t> batched;
+    std::vector<size_t> needs(count, 0);
     for (uint32_t i = 0; i < count; i++) {
         const fg_request* q = &reqs[i];
         if (!q->descs || !q->frames || !q->out || q->frame_count == 0) {
@@ -625,49 +632,65 @@ int fg_batch_fuse(fg_batch* b, uint32_t count, const fg_request* reqs, fg_reques
             note_error(i, vrc);
             continue;
         }
+        needs[i] = need;
         const double roi_px = (double)opts[i].width * (double)opts[i].height;
         const bool batchable = opts[i].method == FG_METHOD_PYRAMID_CONTRAST && (double)need <= group_cap &&
                                !(large_px > 0.0 && roi_px >= large_px);
-        if (!batchable) {
-            // a stack of its own on the next single stream, beside the groups (which keep their order)
-            hipStream_t st = b->streams.size() > 1 ? b->streams[1 + next_single++ % nsingle] : gstream;
-            const double cached_need = (double)need;
-            make_room(std::min(cached_need, b->budget));
-            const double left = b->budget - b->held - inflight_bytes;
-            int c2 = 0;
-            const double sneed = (double)fg_plan_stack_bytes(&opts[i], left, &c2, false);
-            Track& tr = open_track(st);
-            tr.idx = {i};
-            fg_stack* s = nullptr;
-            int rc = start_single(b, q, opts[i], st, left, &s);
-            if (rc == FG_ERR_OOM_DEVICE && !s) {  // alone on the device, as the reference runs it: uncached
-                while (!b->plans.empty()) evict_lru(b);
-                while (!inflight.empty()) retire_oldest();
-                b->arena.overflow = true;
-                rc = start_single(b, q, opts[i], st, b->budget, &s);
-                b->arena.overflow = false;
-            }
-            (void)hipEventRecord(tr.done, st);
-            if (s) {
-                hipEvent_t done = nullptr;
-                (void)hipEventCreateWithFlags(&done, hipEventDisableTiming);
-                (void)hipEventRecord(done, st);
-                inflight.push_back(Single{s, done, sneed});
-                inflight_bytes += sneed;
-            }
-            if (rc == FG_OK)
-                state[i] = FG_REQ_DONE;
-            else
-                note_error(i, rc);
-            continue;
+        if (batchable)
+            batched.push_back(i);
+        else
+            singles.push_back(SingleReq{i, roi_px * (double)q->frame_count});
+    }
+    // stacks of their own first, longest first, each on the side stream with the least work queued
+    // (longest-processing-time order: the batch ends with short stacks, not with one long one alone)
+    std::stable_sort(singles.begin(), singles.end(),
+                     [](const SingleReq& x, const SingleReq& y) { return x.cost > y.cost; });
+    std::vector<double> load(nsingle, 0.0);
+    for (const SingleReq& sr : singles) {
+        const uint32_t i = sr.i;
+        const fg_request* q = &reqs[i];
+        size_t j = 0;
+        for (size_t k = 1; k < nsingle; k++)
+            if (load[k] < load[j]) j = k;
+        load[j] += sr.cost;
+        hipStream_t st = b->streams.size() > 1 ? b->streams[1 + j] : gstream;
+        make_room(std::min((double)needs[i], b->budget));
+        const double left = b->budget - b->held - inflight_bytes;
+        int c2 = 0;
+        const double sneed = (double)fg_plan_stack_bytes(&opts[i], left, &c2, false);
+        Track& tr = open_track(st);
+        tr.idx = {i};
+        fg_stack* s = nullptr;
+        int rc = start_single(b, q, opts[i], st, left, &s);
+        if (rc == FG_ERR_OOM_DEVICE && !s) {  // alone on the device, as the reference runs it: uncached
+            while (!b->plans.empty()) evict_lru(b);
+            while (!inflight.empty()) retire_oldest();
+            b->arena.overflow = true;
+            rc = start_single(b, q, opts[i], st, b->budget, &s);
+            b->arena.overflow = false;
         }
+        (void)hipEventRecord(tr.done, st);
+        if (s) {
+            hipEvent_t done = nullptr;
+            (void)hipEventCreateWithFlags(&done, hipEventDisableTiming);
+            (void)hipEventRecord(done, st);
+            inflight.push_back(Single{s, done, sneed});
+            inflight_bytes += sneed;
+        }
+        if (rc == FG_OK)
+            state[i] = FG_REQ_DONE;
+        else
+            note_error(i, rc);
+    }
+    // the groups, in request order, on the group stream beside them
+    for (uint32_t i : batched) {
         const bool fits = !cur.empty() && opts[i].sample_type == cur_key[0].o.sample_type &&
                           opts[i].contrast_window_size == cur_key[0].o.contrast_window_size &&
-                          cur_bytes + (double)need <= group_cap;
+                          cur_bytes + (double)needs[i] <= group_cap;
         if (!cur.empty() && !fits) flush();
         cur.push_back(i);
         cur_key.push_back(Geom{opts[i]});
-        cur_bytes += (double)need;
+        cur_bytes += (double)needs[i];
     }
     flush();
     hipError_t e = hipSuccess;
