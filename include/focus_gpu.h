@@ -285,10 +285,12 @@ int fg_server_wait(fg_server* server, uint64_t ticket, double timeout_ms, fg_req
  * memory budget together.  A group runs as one persistent pass-1 launch over every stack's tiles, one
  * pass-2 launch per pyramid level and one collapse launch per level, captured as a HIP graph the first
  * time that group geometry (the ordered list of stack shapes) runs and replayed when it recurs (the
- * batch keeps up to plan_cache geometries with their buffers).  Other methods, and stacks too large
- * to cache, run through the plain stack calls.  Every result equals the stack fused alone, and the
- * call returns when every output is written; status (optional) receives one entry per request,
- * with run_ms the device time of its group and total_ms from the call's start to its group's end. */
+ * batch keeps up to plan_cache geometries with their buffers).  Stacks that fill the GPU alone (an
+ * output of at least fg_tuning.batch_large_kpix thousand pixels), other methods and stacks too large
+ * to cache run as stacks of their own (the plain stack calls), longest first, on side streams beside
+ * the groups.  Every result equals the stack fused alone, and the call returns when every output is
+ * written; status (optional) receives one entry per request, with run_ms the device time of its
+ * group (or stack) and total_ms from the call's start to its end. */
 typedef struct fg_batch fg_batch;
 typedef struct fg_batch_opts {
     float memory_fraction;     /* share of free HBM the batch may hold (0 = the context's; <= 0 there: all but 1 GB) */
