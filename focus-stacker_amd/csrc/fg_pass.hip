@@ -1143,6 +1143,31 @@ __device__ __forceinline__ void reduce_row_full(const float* rR, const float* rG
     }
 }
 
+// Level 0 with every tap supported: the rgb samples are integers below 2^16 and the kernel weights
+// sum to 256, so every partial sum of a reduce output is an integer below 2^24, exact in f32, and
+// the sum equals the reference's sequential one in any order.  hsum5: the horizontal (1,4,6,4,1)
+// sum of one LDS row (columns base..base+4); the vertical sums of these give the outputs.
+__device__ __forceinline__ float hsum5_int(const float* c, int base) {
+    const float2 u0 = *(const float2*)&c[base], u1 = *(const float2*)&c[base + 2];
+    const float u4 = (*(const float2*)&c[base + 4]).x;
+    return fmaf(6.0f, u1.x, fmaf(4.0f, u0.y + u1.y, u0.x + u4));
+}
+__device__ __forceinline__ float vsum5_int(float h0, float h1, float h2, float h3, float h4) {
+    return fmaf(6.0f, h2, fmaf(4.0f, h1 + h3, h0 + h4));
+}
+
+// One row of reduce_row_full for a single channel (the mask's sum keeps the reference's order).
+template <int KY>
+__device__ __forceinline__ float reduce_row_full1(const float* c, int base, float a) {
+    const float2 u0 = *(const float2*)&c[base], u1 = *(const float2*)&c[base + 2];
+    const float u4 = (*(const float2*)&c[base + 4]).x;
+    a = red_tap<KY * 1>(a, u0.x);
+    a = red_tap<KY * 4>(a, u0.y);
+    a = red_tap<KY * 6>(a, u1.x);
+    a = red_tap<KY * 4>(a, u1.y);
+    return red_tap<KY * 1>(a, u4);
+}
+
 // w / s correctly rounded from y = RN(1/s) (Markstein correction); the quotient of a tiny
 // numerator, where the correction's residual could leave the normal range, is divided directly.
 __device__ __forceinline__ float div_by_recip(float w, float s, float y) {
@@ -1165,8 +1190,7 @@ __device__ __forceinline__ void pass2_tile(const Pass2Args& a, const LevelArgs& 
     __shared__ __align__(16) float rG[NREG];
     __shared__ __align__(16) float rB[NREG];
     __shared__ __align__(16) float rM[NREG];
-    __shared__ float sS[L0 ? NREG : 1];        // norm sum, -1 where no frame has support
-    __shared__ float sY[L0 ? NREG : 1];        // RN(1 / norm sum)
+    __shared__ float2 sSY[L0 ? NREG : 1];      // norm sum (-1 where no frame has support), RN(1 / norm sum)
     __shared__ float nR[NH * NW], nG[NH * NW], nB[NH * NW];   // level-(l+1) gaussian (values)
     __shared__ float hR[NH * TW], hG[NH * TW], hB[NH * TW];   // its horizontal expansion per tile column
     __shared__ int sPart[2];                   // per frame parity: some region pixel lacks support
@@ -1210,8 +1234,7 @@ __device__ __forceinline__ void pass2_tile(const Pass2Args& a, const LevelArgs& 
                     y = 1.0f / v;
                 }
             }
-            sS[idx] = v;
-            sY[idx] = y;
+            sSY[idx] = make_float2(v, y);
         }
     }
 
@@ -1267,8 +1290,9 @@ __device__ __forceinline__ void pass2_tile(const Pass2Args& a, const LevelArgs& 
                         v.x = Pack<T>::ch(q, 3) > 0 ? r : -(r + 1.0f);
                         v.y = (float)Pack<T>::ch(q, 1);
                         v.z = (float)Pack<T>::ch(q, 2);
-                        const float sv = sS[idx];
-                        v.w = sv < 0.0f ? -1.0f : (sv > 1e-12f ? div_by_recip(pf_w[j], sv, sY[idx]) : m_unsup);
+                        const float2 sy = sSY[idx];
+                        const float sv = sy.x;
+                        v.w = sv < 0.0f ? -1.0f : (sv > 1e-12f ? div_by_recip(pf_w[j], sv, sy.y) : m_unsup);
                     } else {
                         v = pf_v[j];
                     }
@@ -1294,7 +1318,33 @@ __device__ __forceinline__ void pass2_tile(const Pass2Args& a, const LevelArgs& 
                 const int oi = p % NW, oj = 2 * (p / NW);
                 const int base0 = (2 * oj) * RW + 2 * oi;
                 float a0, a1, a2, aw, am, amw, b0, b1, b2, bw, bm, bmw;
-                if (full) {
+                if (L0 && full) {
+                    // every tap in bounds and supported: weight sums are exactly 256; rgb in the
+                    // separable integer form (exact), the mask in the reference's order
+                    float ca[3], cb[3];
+                    const float* rgb[3] = {rR, rG, rB};
+#pragma unroll
+                    for (int c = 0; c < 3; c++) {
+                        float h[7];
+#pragma unroll
+                        for (int rr = 0; rr < 7; rr++) h[rr] = hsum5_int(rgb[c], base0 + rr * RW);
+                        ca[c] = vsum5_int(h[0], h[1], h[2], h[3], h[4]);
+                        cb[c] = vsum5_int(h[2], h[3], h[4], h[5], h[6]);
+                    }
+                    float ma = 0.0f, mb = 0.0f;
+                    ma = reduce_row_full1<1>(rM, base0, ma);
+                    ma = reduce_row_full1<4>(rM, base0 + RW, ma);
+                    ma = reduce_row_full1<6>(rM, base0 + 2 * RW, ma);
+                    mb = reduce_row_full1<1>(rM, base0 + 2 * RW, mb);
+                    ma = reduce_row_full1<4>(rM, base0 + 3 * RW, ma);
+                    mb = reduce_row_full1<4>(rM, base0 + 3 * RW, mb);
+                    ma = reduce_row_full1<1>(rM, base0 + 4 * RW, ma);
+                    mb = reduce_row_full1<6>(rM, base0 + 4 * RW, mb);
+                    mb = reduce_row_full1<4>(rM, base0 + 5 * RW, mb);
+                    mb = reduce_row_full1<1>(rM, base0 + 6 * RW, mb);
+                    a0 = ca[0]; a1 = ca[1]; a2 = ca[2]; am = ma; aw = amw = 256.0f;
+                    b0 = cb[0]; b1 = cb[1]; b2 = cb[2]; bm = mb; bw = bmw = 256.0f;
+                } else if (full) {
                     // every tap in bounds and supported: weight sums are exactly 256
                     float sa[4] = {0.0f, 0.0f, 0.0f, 0.0f}, sb[4] = {0.0f, 0.0f, 0.0f, 0.0f};
                     reduce_row_full<1>(rR, rG, rB, rM, base0, sa);
