@@ -1250,21 +1250,22 @@ __device__ __forceinline__ void pass2_tile(const Pass2Args& a, const LevelArgs& 
     PT pf_px[L0 ? NPF : 1];
     float pf_w[L0 ? NPF : 1];
     float4 pf_v[L0 ? 1 : NPF];
+    // Loads without branches: a pixel outside the level loads element 0 of the same frame (its value
+    // is never used), and the address is the frame's base (wave-uniform) plus a 32-bit byte offset
+    // (levels hold fewer than 2^28 pixels).
     auto prefetch = [&](int k) {
         const size_t slot = (size_t)(k - a.slot_base);
-        const PT* rg = L0 ? (const PT*)a.rgba + slot * (size_t)n : nullptr;
-        const float* wts = L0 ? a.weights + slot * (size_t)n : nullptr;
-        const float4* lv = L0 ? nullptr : a.lev + slot * a.lev_stride + g.lev_off;
+        const char* rg = L0 ? (const char*)((const PT*)a.rgba + slot * (size_t)n) : nullptr;
+        const char* wts = L0 ? (const char*)(a.weights + slot * (size_t)n) : nullptr;
+        const char* lv = L0 ? nullptr : (const char*)(a.lev + slot * a.lev_stride + g.lev_off);
 #pragma unroll
         for (int j = 0; j < NPF; j++) {
-            const int i = gidx[j];
-            if (i >= 0) {
-                if (L0) {
-                    pf_px[j] = rg[i];
-                    pf_w[j] = wts[i];
-                } else {
-                    pf_v[j] = lv[i];
-                }
+            const uint32_t e = gidx[j] >= 0 ? (uint32_t)gidx[j] : 0u;
+            if (L0) {
+                pf_px[j] = *(const PT*)(rg + e * (uint32_t)sizeof(PT));
+                pf_w[j] = *(const float*)(wts + e * 4u);
+            } else {
+                pf_v[j] = *(const float4*)(lv + e * 16u);
             }
         }
     };
@@ -1290,9 +1291,15 @@ __device__ __forceinline__ void pass2_tile(const Pass2Args& a, const LevelArgs& 
                         v.x = Pack<T>::ch(q, 3) > 0 ? r : -(r + 1.0f);
                         v.y = (float)Pack<T>::ch(q, 1);
                         v.z = (float)Pack<T>::ch(q, 2);
+                        // w / S by the Markstein correction from RN(1/S) (div_by_recip), formed
+                        // without branches; 1/N where S <= 1e-12, -1 without support
                         const float2 sy = sSY[idx];
-                        const float sv = sy.x;
-                        v.w = sv < 0.0f ? -1.0f : (sv > 1e-12f ? div_by_recip(pf_w[j], sv, sy.y) : m_unsup);
+                        const float sv = sy.x, w = pf_w[j];
+                        const float qd = w * sy.y;
+                        float mw = fmaf(fmaf(-qd, sv, w), sy.y, qd);
+                        if (w != 0.0f && fabsf(w) < 1e-30f) mw = w / sv;  // tiny numerators divide directly
+                        mw = sv > 1e-12f ? mw : m_unsup;
+                        v.w = sv < 0.0f ? -1.0f : mw;
                     } else {
                         v = pf_v[j];
                     }
