@@ -220,3 +220,63 @@ def test_u8_workload_hardmask_softmask_match_oracle(ctx, fg, gen_u8):
             assert np.array_equal(out, ref), f"{int((out != ref).any(axis=2).sum())} winners differ"
         else:
             _img_close(out, ref)
+
+
+# --------------------------------------------------------------------------
+# the whole 143-frame u16 workload
+# --------------------------------------------------------------------------
+
+@pytest.fixture(scope="module")
+def gen_u16_full(fg):
+    """All 143 frames of the u16 three-plane workload, 7952x5304, left on the GPU (36 GB)."""
+    import torch
+    import synth
+    spec = synth.config_spec(2)
+    print(f"generating {spec.frames} frames of the 7952x5304 u16 workload on the GPU", flush=True)
+    st = synth.Stack(spec, device=torch.device("cuda:0"))
+    frames = []
+    for k in range(spec.frames):
+        f = st.frame(k)
+        frames.append((f - (f > 32767).to(torch.int32) * 65536).to(torch.int16).contiguous())
+    poses = [fg.pose(p) for p in st.poses]
+    del st
+    torch.cuda.synchronize()
+    torch.cuda.empty_cache()
+    roi = fg.common_overlap_roi([(spec.width, spec.height)] * spec.frames, poses)
+    yield frames, poses, roi
+    del frames
+    torch.cuda.empty_cache()
+
+
+@pytest.mark.parametrize("method", ["pyramid", "hybrid"])
+def test_u16_workload_143_frames_cached_equals_uncached(ctx, fg, gen_u16_full, method):
+    """The whole 143-frame stack, streamed: with every frame cached (the benchmark's form: the
+    hybrid keeps its winners as frame slots 0..142) and under a budget too small for the frame cache
+    (the reference's uncached second loop, every frame fed twice; the hybrid keeps winner colours).
+    Both write the same image, and the norm sum and union support agree."""
+    import torch
+    frames, poses, roi = gen_u16_full
+    w, h = roi[2] - roi[0], roi[3] - roi[1]
+    m = fg.METHOD_PYRAMID if method == "pyramid" else fg.METHOD_HYBRID
+    desc = fg.ImageDesc(W, H, 3, 0, fg.SAMPLE_U16)
+    outs, sums = {}, {}
+    for name, frac in (("cached", 0.8), ("uncached", 0.02)):
+        print(f"143 x u16 {method} {name}, roi={roi}", flush=True)
+        s = fg.Stack(ctx, len(frames), w, h, fg.SAMPLE_U16, method=m, sharpness=0.35, memory_fraction=frac)
+        for f, p in zip(frames, poses):
+            s.add_frame_device(f.data_ptr(), desc, p, roi)
+        assert s.needs_pass2_frames() == (name == "uncached")
+        if s.needs_pass2_frames():
+            for f, p in zip(frames, poses):
+                s.pass2_frame_device(f.data_ptr(), desc, p, roi)
+        out = torch.empty((h, w, 3), dtype=torch.int16, device="cuda:0")
+        torch.cuda.synchronize()
+        s.finish_device(out.data_ptr())
+        s.sync()
+        outs[name] = out.cpu().numpy()
+        sums[name] = (s.readback("norm_sum"), s.readback("union_support"))
+        s.close()
+    assert np.array_equal(outs["cached"], outs["uncached"])
+    assert np.array_equal(sums["cached"][0], sums["uncached"][0])
+    assert np.array_equal(sums["cached"][1], sums["uncached"][1])
+    assert int(outs["cached"].view(np.uint16).max()) > 0
