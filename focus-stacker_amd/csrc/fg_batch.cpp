@@ -10,9 +10,10 @@
 //   4. one collapse launch per level (top down) and one that adds level 0 and quantizes.
 // Each stack computes exactly what it computes alone: the same kernels run on the same tiles, with
 // sums in frame order whatever the launch they share.  The schedule of a group geometry (the ordered
-// list of stack shapes) is captured from the batch's stream into a HIP graph on first use and kept
-// with the group's stacks (a plan) for later groups of the same geometry; only the frame
-// descriptors (which frames, which poses) are uploaded again per run.
+// list of stack shapes) is kept with the group's stacks (a plan) for later groups of the same
+// geometry: launched one by one on its first run, captured from the batch's stream into a HIP graph
+// on its second and replayed from then on; only the frame descriptors (which frames, which poses)
+// are uploaded again per run.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -86,6 +87,7 @@ struct Plan {
     std::vector<int> single_level;         // stacks of one level: quantized directly
     hipGraphExec_t graph = nullptr;
     bool eager = false;                    // capture failed once: launch one by one
+    int runs = 0;                          // runs so far: the schedule is captured on the second
     hipStream_t stream = nullptr;          // the stream of its runs (a plan runs on one stream only)
     uint64_t last_use = 0;
     std::vector<void*> tables;             // arena blocks of the tables
@@ -309,8 +311,9 @@ int enqueue_schedule(Plan* p) {
     return FG_OK;
 }
 
-// Runs a plan on the group's requests: frame descriptors up, the schedule (its graph, captured on the
-// plan's first run), the outputs copied to the requests' buffers.
+// Runs a plan on the group's requests: frame descriptors up, the schedule (launched one by one on the
+// plan's first run, captured on its second, its graph replayed after), the outputs copied to the
+// requests' buffers.
 int run_plan(Plan* p, const std::vector<const fg_request*>& reqs) {
     hipStream_t stream = p->stream;
     if (p->staged_pending) {  // the previous run's upload has to have read the staging
@@ -328,7 +331,9 @@ int run_plan(Plan* p, const std::vector<const fg_request*>& reqs) {
         hipEventRecord(p->staged, stream) != hipSuccess)
         return bfail(FG_ERR_DEVICE, "batch frame upload");
     p->staged_pending = true;
-    if (!p->graph && !p->eager) {
+    // a geometry's first run launches its schedule one by one (a batch that never recurs pays for
+    // no capture); the second captures it, and it is replayed from then on
+    if (!p->graph && !p->eager && p->runs > 0) {
         hipGraph_t g = nullptr;
         int rc = FG_OK;
         hipError_t e = hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal);
@@ -351,6 +356,7 @@ int run_plan(Plan* p, const std::vector<const fg_request*>& reqs) {
         const int rc = enqueue_schedule(p);
         if (rc) return rc;
     }
+    p->runs++;
     for (size_t i = 0; i < reqs.size(); i++) {
         const fg::StackView& v = p->views[i];
         const size_t bytes = v.px * 3 * (size_t)v.sb;

@@ -283,8 +283,8 @@ int fg_server_wait(fg_server* server, uint64_t ticket, double timeout_ms, fg_req
  * requests (device frames and outputs, as fg_server_submit does) and cuts them, in order, into
  * groups of pyramid-contrast stacks of one sample type and contrast window whose frame caches fit the
  * memory budget together.  A group runs as one persistent pass-1 launch over every stack's tiles, one
- * pass-2 launch per pyramid level and one collapse launch per level, captured as a HIP graph the first
- * time that group geometry (the ordered list of stack shapes) runs and replayed when it recurs (the
+ * pass-2 launch per pyramid level and one collapse launch per level, captured as a HIP graph the second
+ * time that group geometry (the ordered list of stack shapes) runs and replayed from then on (the
  * batch keeps up to plan_cache geometries with their buffers).  Stacks that fill the GPU alone (an
  * output of at least fg_tuning.batch_large_kpix thousand pixels), other methods and stacks too large
  * to cache run as stacks of their own (the plain stack calls), longest first, on side streams beside

@@ -105,8 +105,8 @@ def test_batch_of_uneven_stacks_equals_each_alone(ctx, fg):
 
 
 def test_batch_replays_its_graph_with_other_frames(ctx, fg):
-    """The second run of a group geometry replays the schedule captured on the first; only the frame
-    descriptors change.  Frames reversed and mirrored give a different stack of the same shape."""
+    """The second run of a group geometry captures its schedule and the third replays it; only the
+    frame descriptors change.  Frames reversed and mirrored give a different stack of the same shape."""
     import torch
     stacks = _mix(fg, 2, 5, 8)
     outs = [_out(fg, s) for s in stacks]
