@@ -340,6 +340,7 @@ int check_desc(const fg_image_desc* d) {
     if (d->extra_channels > 1) return fail(FG_ERR_UNSUPPORTED, "extra_channels must be 0 or 1");
     if (d->sample_type != FG_SAMPLE_U8 && d->sample_type != FG_SAMPLE_U16)
         return fail(FG_ERR_UNSUPPORTED, "sample_type must be u8 or u16");
+    if (frame_too_large(*d)) return fail(FG_ERR_UNSUPPORTED, "frames must hold fewer than 2^32 - 16 bytes");
     return FG_OK;
 }
 
@@ -728,6 +729,14 @@ static StackPlan plan_stack(const fg_stack_opts& o, double budget, bool host_fra
     return pl;
 }
 
+// The kernels address a frame's samples, and a stack's pixels at every level (16 bytes each for the
+// level buffers), with 32-bit byte offsets.
+bool frame_too_large(const fg_image_desc& d) {
+    const uint64_t sb = d.sample_type == FG_SAMPLE_U16 ? 2 : 1;
+    return (uint64_t)d.width * d.height * (d.color_channels + d.extra_channels) * sb >= (1ull << 32) - 16;
+}
+constexpr uint64_t kMaxStackPixels = 1ull << 28;
+
 static int check_stack_opts(const fg_stack_opts* o) {
     if (o->method > FG_METHOD_HYBRID_PYRAMID_CONTRAST) return fail(FG_ERR_UNSUPPORTED, "unknown fusion method");
     if (o->contrast_window_size < 3 || (o->contrast_window_size & 1) == 0 || o->contrast_window_size > 31)
@@ -735,6 +744,8 @@ static int check_stack_opts(const fg_stack_opts* o) {
     if (!(o->hybrid_sharpness >= 0.0f && o->hybrid_sharpness <= 1.0f))
         return fail(FG_ERR_INVALID_ARG, "hybrid_sharpness must be in [0, 1]");
     if (o->frame_count == 0 || o->width == 0 || o->height == 0) return fail(FG_ERR_INVALID_ARG, "empty stack");
+    if ((uint64_t)o->width * o->height >= kMaxStackPixels)
+        return fail(FG_ERR_UNSUPPORTED, "stacks must hold fewer than 2^28 pixels");
     if (o->sample_type != FG_SAMPLE_U8 && o->sample_type != FG_SAMPLE_U16)
         return fail(FG_ERR_UNSUPPORTED, "sample_type must be u8 or u16");
     return FG_OK;

@@ -394,6 +394,7 @@ int validate_request(const fg_request* q, const fg_stack_opts& o) {
         if (d.color_channels != 3) return bfail(FG_ERR_UNSUPPORTED, "pyramid fusion needs rgb frames");
         if (d.extra_channels > 1) return bfail(FG_ERR_UNSUPPORTED, "extra_channels must be 0 or 1");
         if (d.sample_type != d0.sample_type) return bfail(FG_ERR_FORMAT_MISMATCH, "frame sample types differ");
+        if (frame_too_large(d)) return bfail(FG_ERR_UNSUPPORTED, "frames must hold fewer than 2^32 - 16 bytes");
         if (d.width != d0.width || d.height != d0.height)
             return bfail(FG_ERR_SIZE_MISMATCH, "frame size differs from the first frame's");
         if (d.extra_channels != d0.extra_channels) return bfail(FG_ERR_FORMAT_MISMATCH, "frame channels differ");

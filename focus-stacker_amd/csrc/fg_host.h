@@ -41,5 +41,7 @@ void fg_noalloc_setup(int device);
 // fg_stack_destroy for a stack whose work is known complete: does not wait on its stream (which
 // may be shared with, and busy running, another stack).
 void fg_stack_destroy_idle(fg_stack* stack);
+// A frame too large for the kernels' 32-bit byte offsets into its samples.
+bool frame_too_large(const fg_image_desc& d);
 // Sets the calling thread's fg_last_error() message and returns `code`.
 int fg_set_error(int code, const char* msg);

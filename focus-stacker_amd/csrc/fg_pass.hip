@@ -152,6 +152,23 @@ __device__ __forceinline__ float div_markstein(float a, float b, float y) {
     return fmaf(r, y, q);
 }
 
+// sqrt(x), correctly rounded, for normal x >= 2^-767: an rsq estimate refined by one Goldschmidt
+// step (s ~ sqrt(x), h ~ 1/(2 sqrt(x))) and two Newton corrections of s by its residual x - s^2.
+// These are the operations the compiler's own f64 sqrt runs, which first rescales inputs below
+// 2^-767 by 2^256 (and the result back) and passes 0 and inf through; for x >= 2^-767 the rescale is
+// by 2^0, so the bits are the same.
+__device__ __forceinline__ double sqrt_normal(double x) {
+    const double g = __builtin_amdgcn_rsq(x);
+    double s = x * g, h = g * 0.5;
+    const double r = fma(-h, s, 0.5);
+    s = fma(s, r, s);
+    h = fma(h, r, h);
+    double d = fma(-s, s, x);
+    s = fma(d, h, s);
+    d = fma(-s, s, x);
+    return fma(d, h, s);
+}
+
 // LDS-only workgroup barrier: orders LDS accesses across the block without waiting for the
 // block's outstanding global stores (no thread reads another thread's global stores in a launch).
 __device__ __forceinline__ void lds_barrier() {
@@ -372,11 +389,10 @@ __device__ __forceinline__ bool bilinear_taps(const uint32_t t[12], double fx, d
 #pragma unroll
     for (int c = 0; c < 3; c++) {
         const double v = (((double)t[c] * w00 + (double)t[3 + c] * w10) + (double)t[6 + c] * w01) + (double)t[9 + c] * w11;
-        const double q = v * inv + 0.5;
-        const double fl = floor(q);
-        const double d = q - fl;
+        const double q = v * inv + 0.5;  // > 0: truncation is the floor
+        const double d = __builtin_amdgcn_fract(q);
         ok = ok && d > band && d < 1.0 - band;
-        o[c] = (uint32_t)fl;
+        o[c] = (uint32_t)q;
     }
     return ok;
 }
@@ -395,10 +411,9 @@ __device__ __forceinline__ bool bilinear_taps_u8(const uint32_t t[12], double fx
         const float p00 = (float)t[c], p10 = (float)t[3 + c], p01 = (float)t[6 + c], p11 = (float)t[9 + c];
         const float top = fmaf(fx32, p10 - p00, p00);
         const float bot = fmaf(fx32, p11 - p01, p01);
-        const float q = fmaf(fy32, bot - top, top) + 0.5f;
-        const float fl = floorf(q);
-        near = near || fabsf((q - fl) - 0.5f) > 0.5f - 2.5e-4f;
-        o[c] = (uint32_t)fl;
+        const float q = fmaf(fy32, bot - top, top) + 0.5f;  // > 0: truncation is the floor
+        near = near || fabsf(__builtin_amdgcn_fractf(q) - 0.5f) > 0.5f - 2.5e-4f;
+        o[c] = (uint32_t)q;
     }
     return near ? bilinear_taps(t, fx, fy, band, o) : true;
 }
@@ -418,11 +433,10 @@ __device__ __forceinline__ bool bilinear_taps_lerp64(const uint32_t t[12], doubl
         const double d0 = (double)((int)t[3 + c] - (int)t[c]), d1 = (double)((int)t[9 + c] - (int)t[6 + c]);
         const double top = fma(fx, d0, p00);
         const double bot = fma(fx, d1, p01);
-        const double q = fma(fy, bot - top, top) + 0.5;
-        const double fl = floor(q);
-        const double d = q - fl;
+        const double q = fma(fy, bot - top, top) + 0.5;  // > 0: truncation is the floor
+        const double d = __builtin_amdgcn_fract(q);
         ok = ok && fabs(d - 0.5) < 0.5 - band;
-        o[c] = (uint32_t)fl;
+        o[c] = (uint32_t)q;
     }
     return ok;
 }
@@ -435,7 +449,7 @@ __device__ __forceinline__ bool bilinear_taps_lerp64(const uint32_t t[12], doubl
 template <typename T, bool STAGED, bool INSIDE = false>
 __device__ __forceinline__ bool remap_fast(const FastFrame& ff, const ColHomog& ch, const StageBox& s,
                                            const uint8_t* stage, uint32_t lead, const T* __restrict__ src, int gy,
-                                           double band, uint32_t o[3]) {
+                                           double band, uint32_t o[3], double* seed = nullptr) {
     // INSIDE (staged tiles whose whole footprint is in the source interior and the box, StageBox::
     // inside): the checks below hold for every pixel and are skipped
     // the reference's c + (f * (l / denom) + shift) with the focal length folded into the homography
@@ -445,12 +459,33 @@ __device__ __forceinline__ bool remap_fast(const FastFrame& ff, const ColHomog& 
     const double ly = fma(ff.b4f, pyd, ch.cy);
     const double denom = -fma(ff.b7, pyd, ch.cz);
     if (!INSIDE && !(ff.legacy ? denom > 1e-6 : fabs(denom) > 1e-6)) return false;
-    const double inv = rcp_f64(denom);
+    double inv;
+    if constexpr (INSIDE) {
+        // 1/denom by two Newton steps from the reciprocal of this column's previous row (`seed`; the
+        // denominator is affine in the row and changes little between them): the seed's relative
+        // error squares at each step, so from 2^-20 or less the result is as accurate as
+        // rcp_f64's.  A wave where some lane's seed is further off (the frame's first row, strong
+        // perspective) starts from the hardware estimate instead.
+        double r = *seed;
+        double e = fma(-denom, r, 1.0);
+        if (__builtin_amdgcn_ballot_w64(!(fabs(e) <= 0x1p-20))) {
+            asm volatile("" ::: "memory");  // a real branch: the estimate is not speculated
+            r = __builtin_amdgcn_rcp(denom);
+            e = fma(-denom, r, 1.0);
+        }
+        r = fma(r, e, r);
+        e = fma(-denom, r, 1.0);
+        inv = fma(r, e, r);
+        *seed = inv;
+    } else {
+        inv = rcp_f64(denom);
+    }
     const double x = fma(lx, inv, ff.sx0);
     const double y = fma(ly, inv, ff.sy0);
     if (!INSIDE && !(x >= 0.0 && y >= 0.0 && x < ff.wlim && y < ff.hlim)) return false;
-    const double fx0 = floor(x), fy0 = floor(y);
-    const int x0 = (int)fx0, y0 = (int)fy0;
+    // x, y >= 0 here: truncation is the floor, and fract(x) = x - floor(x) exactly
+    const int x0 = (int)x, y0 = (int)y;
+    const double fx = __builtin_amdgcn_fract(x), fy = __builtin_amdgcn_fract(y);
     uint32_t t[12];
     if (STAGED) {
         if (!INSIDE && (x0 < s.x_lo || y0 < s.y_lo || y0 + 1 >= s.y_lo + s.rows)) return false;
@@ -481,8 +516,8 @@ __device__ __forceinline__ bool remap_fast(const FastFrame& ff, const ColHomog& 
             t[6 + c] = px[b00 + rs + c];
         }
     }
-    if (sizeof(T) == 1) return bilinear_taps_u8(t, x - fx0, y - fy0, band, o);
-    return bilinear_taps_lerp64(t, x - fx0, y - fy0, band, o);
+    if (sizeof(T) == 1) return bilinear_taps_u8(t, fx, fy, band, o);
+    return bilinear_taps_lerp64(t, fx, fy, band, o);
 }
 
 // Cross-lane moves within a wave (DPP wave_shr:1 / wave_shl:1): the value of lane - 1 / lane + 1,
@@ -677,11 +712,12 @@ __device__ __forceinline__ void pass1_tile(const Pass1Args& a, const int bx, con
             }
             const PT c = P::make(o[0], o[1], o[2], o[3]);
             if (keep_rgba && r >= b && r < RH - b && lane >= b && lane < P1C - b && (IN || (gx < W && gy < H))) {
-                const size_t i = (size_t)gy * W + gx;  // an output pixel of the tile
+                // an output pixel of the tile; 32-bit byte offsets (frames hold fewer than 2^28 pixels)
+                const uint32_t i = (uint32_t)gy * (uint32_t)W + (uint32_t)gx;
                 if (sizeof(T) == 1)
-                    ((uint32_t*)rgba)[i] = *(const uint32_t*)&c;
+                    *(uint32_t*)((char*)rgba + 4u * i) = *(const uint32_t*)&c;
                 else
-                    ((uint2*)rgba)[i] = *(const uint2*)&c;
+                    *(uint2*)((char*)rgba + 8u * i) = *(const uint2*)&c;
             }
             gray[r * P1C + lane] = g;
             alph[r * P1C + lane] = (AT)o[3];
@@ -693,11 +729,12 @@ __device__ __forceinline__ void pass1_tile(const Pass1Args& a, const int bx, con
         // loop, the one call site, which keeps the call out of the loops
         uint32_t redo = 0;
         if (box.ok) {  // staged: no per-pixel checks
+            double seed = 0.0;  // the previous row's reciprocal denominator (none yet)
 #pragma unroll 1
             for (int r = wave, i = 0; r < RH; r += 4, i++) {
                 const int gy = Y0 - b + r;
                 uint32_t o[4] = {0, 0, 0, SampleMax<T>::i};
-                if (!remap_fast<T, true, true>(ff, chm, box, (const uint8_t*)stage, lead, fsrc, gy, band, o))
+                if (!remap_fast<T, true, true>(ff, chm, box, (const uint8_t*)stage, lead, fsrc, gy, band, o, &seed))
                     redo |= 1u << i;
                 emit(r, gy, o, std::true_type{});
             }
@@ -808,15 +845,21 @@ __device__ __forceinline__ void pass1_tile(const Pass1Args& a, const int bx, con
                         m = (sw * sw) / nf;
                         var = (qw - m) / den;
                     }
-                    if (!(var > 0.0)) var = 0.0;
-                    stdw = (float)sqrt(var);
+                    // a positive variance of 8/16-bit gray windows is far above 2^-767 (the
+                    // smallest is about 2^-90); a wave holding one below takes the library's sqrt
+                    if (__builtin_amdgcn_ballot_w64(var > 0.0 && var < 0x1p-767)) {
+                        asm volatile("" ::: "memory");  // a real branch: never speculated
+                        stdw = var > 0.0 ? (float)sqrt(var) : 0.0f;
+                    } else {
+                        stdw = var > 0.0 ? (float)sqrt_normal(var) : 0.0f;
+                    }
                 }
                 const uint32_t alpha = FULL ? SampleMax<T>::i : (uint32_t)alph[(r + bb) * P1C + lane];
                 const float af = FULL || alpha == SampleMax<T>::i
                                      ? 1.0f
                                      : div_markstein((float)alpha, SampleMax<T>::f, 1.0f / SampleMax<T>::f);
                 const float wv = stdw * af;  // applySupportInto
-                if (keep_w) wout[i] = wv;
+                if (keep_w) *(float*)((char*)wout + 4u * ((uint32_t)y * (uint32_t)W + (uint32_t)ox)) = wv;
                 const int t = r * P1C + lane;
                 if (acc) {
                     tS[t] = tS[t] + wv;
@@ -1297,7 +1340,13 @@ __device__ __forceinline__ void pass2_tile(const Pass2Args& a, const LevelArgs& 
                         const float sv = sy.x, w = pf_w[j];
                         const float qd = w * sy.y;
                         float mw = fmaf(fmaf(-qd, sv, w), sy.y, qd);
-                        if (w != 0.0f && fabsf(w) < 1e-30f) mw = w / sv;  // tiny numerators divide directly
+                        // tiny numerators divide directly, behind a wave-uniform branch: weights
+                        // are stds of 8/16-bit gray values, far above 1e-30 where nonzero
+                        const bool tiny = w != 0.0f && fabsf(w) < 1e-30f;
+                        if (__builtin_amdgcn_ballot_w64(tiny)) {
+                            asm volatile("" ::: "memory");  // a real branch: the division is not speculated
+                            if (tiny) mw = w / sv;
+                        }
                         mw = sv > 1e-12f ? mw : m_unsup;
                         v.w = sv < 0.0f ? -1.0f : mw;
                     } else {
@@ -1425,16 +1474,20 @@ __device__ __forceinline__ void pass2_tile(const Pass2Args& a, const LevelArgs& 
                 const int gj = oy1 + r;
                 float h0 = 0.0f, h1 = 0.0f, h2 = 0.0f;
                 if (inner) {  // horizontalExpandRgbRow without the clamps it cannot need here
+                    // even and odd columns interleave in a wave: one form for both, without
+                    // branches.  Even: ((a + 6b) + c) / 8; odd: ((b + c) + -0) / 2, where adding -0
+                    // returns the sum unchanged (its sign of zero included)
                     const int c1 = r * NW - ox1 + (x >> 1);
-                    if ((x & 1) == 0) {
-                        h0 = ((nR[c1 - 1] + 6.0f * nR[c1]) + nR[c1 + 1]) * 0.125f;
-                        h1 = ((nG[c1 - 1] + 6.0f * nG[c1]) + nG[c1 + 1]) * 0.125f;
-                        h2 = ((nB[c1 - 1] + 6.0f * nB[c1]) + nB[c1 + 1]) * 0.125f;
-                    } else {
-                        h0 = (nR[c1] + nR[c1 + 1]) * 0.5f;
-                        h1 = (nG[c1] + nG[c1 + 1]) * 0.5f;
-                        h2 = (nB[c1] + nB[c1 + 1]) * 0.5f;
-                    }
+                    const bool ev = (x & 1) == 0;
+                    const float sc = ev ? 0.125f : 0.5f;
+                    auto hx = [&](const float* nv) {
+                        const float a = nv[c1 - 1], b = nv[c1], c = nv[c1 + 1];
+                        const float u = ev ? a : b, v = ev ? 6.0f * b : c, w = ev ? c : -0.0f;
+                        return ((u + v) + w) * sc;
+                    };
+                    h0 = hx(nR);
+                    h1 = hx(nG);
+                    h2 = hx(nB);
                 } else if (x < g.w && gj >= 0 && gj < g.nh) {
                     const int i = x >> 1;
                     const int rb = r * NW - ox1;

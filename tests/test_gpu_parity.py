@@ -534,9 +534,20 @@ def test_errors_surface_as_status_codes(ctx, fg):
     with pytest.raises(fg.FgError) as e:
         s.finish()
     assert e.value.code == -8  # FG_ERR_STATE
+    # the kernels address frames and levels with 32-bit byte offsets: larger ones are refused
+    # before anything runs
+    import torch
+    dummy = torch.zeros(16, dtype=torch.uint8, device="cuda:0")
+    with pytest.raises(fg.FgError) as e:
+        s.add_frame_device(dummy.data_ptr(), fg.ImageDesc(40000, 40000, 3, 0, fg.SAMPLE_U8),
+                           fg.pose(), (0, 0, 32, 24))
+    assert e.value.code == -6  # FG_ERR_UNSUPPORTED
     s.close()
     with pytest.raises(fg.FgError):
         fg.Stack(ctx, 2, 32, 24, window=4)
+    with pytest.raises(fg.FgError) as e:
+        fg.Stack(ctx, 2, 16384, 16384)
+    assert e.value.code == -6  # FG_ERR_UNSUPPORTED: 2^28 pixels
 
 
 SINGLE_CASES = {
