@@ -361,6 +361,18 @@ __device__ __forceinline__ void stage_store(const StageBox& s, int lane, int wav
     }
 }
 
+// Frame descriptor k, read through the constant address space: the host writes the table before the
+// launch and no kernel writes it, so its uniform fields load as scalars (s_load, counted by
+// lgkmcnt).  Through a global pointer they would be vector loads, and waiting for them (vmcnt)
+// would also wait for every global store the wave has in flight.
+__device__ __forceinline__ FrameDesc frame_desc(const FrameDesc* frames, int k) {
+#if defined(__HIP_DEVICE_COMPILE__)  // the host pass of this file has no constant address space
+    return ((const __attribute__((address_space(4))) FrameDesc*)frames)[k];
+#else
+    return frames[k];
+#endif
+}
+
 // A frame's fast-path constants, copied out of its descriptor once per frame (the descriptor sits
 // in global memory, where the kernel's own stores could alias it for the compiler).
 struct FastFrame {
@@ -625,12 +637,16 @@ __device__ __forceinline__ void pass1_tile(const Pass1Args& a, const int bx, con
     // corner c (threads 0-3) of the region in frame k's source, NaN where not staged
     auto corner = [&](int k) {
         if (threadIdx.x >= 4) return;
+        // the corner's position is formed anew for every frame: held across the frame loop, it was
+        // spilled, and its reload waited (vmcnt) for every store of wave 0 before the barrier
+        int tid = (int)threadIdx.x;
+        asm volatile("" : "+v"(tid));
         double x = __builtin_nan(""), y = __builtin_nan(""), d = 0.0;
         if (k < a.k1) {
-            const FrameDesc& f = a.frames[k];
+            const FrameDesc f = frame_desc(a.frames, k);
             if (f.kind == 0 && f.p.mode == REMAP_HOMOGRAPHY) {
-                const int cx = X0 - b + ((threadIdx.x & 1) ? P1C - 1 : 0);
-                const int cy = Y0 - b + ((threadIdx.x & 2) ? RH - 1 : 0);
+                const int cx = X0 - b + ((tid & 1) ? P1C - 1 : 0);
+                const int cy = Y0 - b + ((tid & 2) ? RH - 1 : 0);
                 double sx, sy;
                 if (fast_coord(f.p, col_homog(f.p, cx), cy, &sx, &sy, &d)) {
                     x = sx;
@@ -638,9 +654,9 @@ __device__ __forceinline__ void pass1_tile(const Pass1Args& a, const int bx, con
                 }
             }
         }
-        corners[2 * threadIdx.x] = x;
-        corners[2 * threadIdx.x + 1] = y;
-        cden[threadIdx.x] = d;
+        corners[2 * tid] = x;
+        corners[2 * tid + 1] = y;
+        cden[tid] = d;
     };
 
 #pragma unroll 1
@@ -677,16 +693,17 @@ __device__ __forceinline__ void pass1_tile(const Pass1Args& a, const int bx, con
     // (StageBox::inside): the staged remap loop then needs no per-pixel checks.  The other tiles
     // (a border of the stack, a source edge) gather their taps from global memory.
     const bool region_in = X0 - b >= 0 && Y0 - b >= 0 && X0 - b + P1C <= W && Y0 - b + RH <= H;
-    StageBox box = stage_box<T>(corners, cden, a.frames[a.k0]);
+    const FrameDesc f0 = frame_desc(a.frames, a.k0);
+    StageBox box = stage_box<T>(corners, cden, f0);
     box.ok = box.ok && box.inside && region_in;
     if (box.ok) {
-        stage_load<T>(a.frames[a.k0], box, lane, wave, pf);
+        stage_load<T>(f0, box, lane, wave, pf);
         stage_store<T>(box, lane, wave, pf, stage);
     }
     lds_barrier();
 
     for (int k = a.k0; k < a.k1; k++) {
-        const FrameDesc& f = a.frames[k];
+        const FrameDesc f = frame_desc(a.frames, k);
         const size_t slot = (size_t)(k - a.slot_base);
         T* rgba = keep_rgba ? (T*)a.rgba + slot * fpx * 4 : nullptr;
         float* wout = keep_w ? a.weights + slot * fpx : nullptr;
@@ -771,9 +788,10 @@ __device__ __forceinline__ void pass1_tile(const Pass1Args& a, const int bx, con
         lds_barrier();
         const bool full = contrast_ok && sFull[0] && sFull[1] && sFull[2] && sFull[3];
         // the next frame's source box, in flight until the end of this frame
-        StageBox nbox = k + 1 < a.k1 ? stage_box<T>(corners, cden, a.frames[k + 1]) : StageBox{0, 0, 0, 0, 0, 0, 0};
+        const FrameDesc fn = frame_desc(a.frames, k + 1 < a.k1 ? k + 1 : k);
+        StageBox nbox = k + 1 < a.k1 ? stage_box<T>(corners, cden, fn) : StageBox{0, 0, 0, 0, 0, 0, 0};
         nbox.ok = nbox.ok && nbox.inside && region_in;
-        if (nbox.ok) stage_load<T>(a.frames[k + 1], nbox, lane, wave, pf);
+        if (nbox.ok) stage_load<T>(fn, nbox, lane, wave, pf);
         // contrast of this wave's output rows: window-column sums slide down the column in
         // registers (lane = column), the window's columns are added across lanes
         // (computeRange src/fuse/contrast.zig:172-282: f64 sums of x and of the f32 product x*x).
