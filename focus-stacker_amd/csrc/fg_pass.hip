@@ -614,8 +614,8 @@ __device__ __forceinline__ void pass1_tile(const Pass1Args& a, const int bx, con
     uint64_t* tUw = (uint64_t*)(smem + L.tU);         // [th]
     float* tB = (float*)(smem + L.tB);                // [th][64] hybrid: best weight
     uint16_t* tK = (uint16_t*)(smem + L.tK);          // [th][64] hybrid, frames cached: its slot
-    __shared__ double corners[8];                     // next frame's region corners in its source
-    __shared__ double cden[4];                        // the homography denominators there
+    __shared__ double corners[2][8];                  // region corners in the source of frames k, k+1 (by k & 1)
+    __shared__ double cden[2][4];                     // the homography denominators there
     __shared__ int sFull[4];                          // per wave: every region pixel it remapped has full alpha
 
     const int W = (int)a.w, H = (int)a.h;
@@ -654,9 +654,9 @@ __device__ __forceinline__ void pass1_tile(const Pass1Args& a, const int bx, con
                 }
             }
         }
-        corners[2 * tid] = x;
-        corners[2 * tid + 1] = y;
-        cden[tid] = d;
+        corners[k & 1][2 * tid] = x;
+        corners[k & 1][2 * tid + 1] = y;
+        cden[k & 1][tid] = d;
     };
 
 #pragma unroll 1
@@ -688,13 +688,14 @@ __device__ __forceinline__ void pass1_tile(const Pass1Args& a, const int bx, con
     // stage the first frame
     uint32_t pf[G::NR * G::NC];
     corner(a.k0);
+    corner(a.k0 + 1);
     lds_barrier();
     // frames are staged where the region lies inside the stack and its footprint inside the source
     // (StageBox::inside): the staged remap loop then needs no per-pixel checks.  The other tiles
     // (a border of the stack, a source edge) gather their taps from global memory.
     const bool region_in = X0 - b >= 0 && Y0 - b >= 0 && X0 - b + P1C <= W && Y0 - b + RH <= H;
     const FrameDesc f0 = frame_desc(a.frames, a.k0);
-    StageBox box = stage_box<T>(corners, cden, f0);
+    StageBox box = stage_box<T>(corners[a.k0 & 1], cden[a.k0 & 1], f0);
     box.ok = box.ok && box.inside && region_in;
     if (box.ok) {
         stage_load<T>(f0, box, lane, wave, pf);
@@ -703,6 +704,14 @@ __device__ __forceinline__ void pass1_tile(const Pass1Args& a, const int bx, con
     lds_barrier();
 
     for (int k = a.k0; k < a.k1; k++) {
+        // the next frame's source box (its corners were published by the previous barrier): its
+        // loads are issued before this frame's stores and are in flight through the remap, so
+        // waiting for them (before the contrast) does not wait for the contrast's stores
+        const FrameDesc fn = frame_desc(a.frames, k + 1 < a.k1 ? k + 1 : k);
+        StageBox nbox = k + 1 < a.k1 ? stage_box<T>(corners[(k + 1) & 1], cden[(k + 1) & 1], fn)
+                                     : StageBox{0, 0, 0, 0, 0, 0, 0};
+        nbox.ok = nbox.ok && nbox.inside && region_in;
+        if (nbox.ok) stage_load<T>(fn, nbox, lane, wave, pf);
         const FrameDesc f = frame_desc(a.frames, k);
         const size_t slot = (size_t)(k - a.slot_base);
         T* rgba = keep_rgba ? (T*)a.rgba + slot * fpx * 4 : nullptr;
@@ -784,14 +793,12 @@ __device__ __forceinline__ void pass1_tile(const Pass1Args& a, const int bx, con
             const bool wf = __all(lane_full ? 1 : 0);
             if (lane == 0) sFull[wave] = wf ? 1 : 0;
         }
-        corner(k + 1);
+        corner(k + 2);  // its buffer held frame k's corners, read by every wave before the last barrier
         lds_barrier();
         const bool full = contrast_ok && sFull[0] && sFull[1] && sFull[2] && sFull[3];
-        // the next frame's source box, in flight until the end of this frame
-        const FrameDesc fn = frame_desc(a.frames, k + 1 < a.k1 ? k + 1 : k);
-        StageBox nbox = k + 1 < a.k1 ? stage_box<T>(corners, cden, fn) : StageBox{0, 0, 0, 0, 0, 0, 0};
-        nbox.ok = nbox.ok && nbox.inside && region_in;
-        if (nbox.ok) stage_load<T>(fn, nbox, lane, wave, pf);
+        // every wave is done with this frame's staged box: the next one goes in
+        box = nbox;
+        if (box.ok) stage_store<T>(box, lane, wave, pf, stage);
         // contrast of this wave's output rows: window-column sums slide down the column in
         // registers (lane = column), the window's columns are added across lanes
         // (computeRange src/fuse/contrast.zig:172-282: f64 sums of x and of the f32 product x*x).
@@ -908,10 +915,6 @@ __device__ __forceinline__ void pass1_tile(const Pass1Args& a, const int bx, con
             else
                 contrast_rows(std::integral_constant<int, 0>{}, std::false_type{});
         }
-        // the next frame's box: its loads were issued before the contrast, the remap of this
-        // frame is done with the staging (first barrier above)
-        box = nbox;
-        if (box.ok) stage_store<T>(box, lane, wave, pf, stage);
         lds_barrier();
     }
     // the winners' RGBA were stored to the frame cache by whichever wave remapped their row: a
