@@ -305,19 +305,19 @@ __device__ __forceinline__ void stage_load(const FrameDesc& f, const StageBox& s
     // block-uniform: plain dword loads.  16-bit frames only: the 8-bit kernel runs at 128 VGPRs, where
     // this second copy of the loads costs more in register spills than its checks save
     if (sizeof(T) == 2 && s.interior) {
+        // a wave-uniform row pointer and this lane's fixed offset in the row: scalar row arithmetic,
+        // no per-load address computation.  Slots outside the box are left unset: stage_store skips
+        // them under the same conditions.
 #pragma unroll
         for (int ir = 0; ir < G::NR; ir++) {
             const int row = wave + 4 * ir;  // wave-uniform
-            if (row >= s.rows) {
-#pragma unroll
-                for (int ic = 0; ic < G::NC; ic++) pf[ir * G::NC + ic] = 0u;
-                continue;
-            }
-            const uint32_t rw = stage_row_byte<T>(f, s, s.y_lo + row, lead) >> 2;
+            if (row >= s.rows) continue;
+            const __attribute__((address_space(1))) uint32_t* rp =
+                dwp + (stage_row_byte<T>(f, s, s.y_lo + row, lead) >> 2);
 #pragma unroll
             for (int ic = 0; ic < G::NC; ic++) {
                 const int col = lane + 64 * ic;
-                pf[ir * G::NC + ic] = col < s.dw ? dwp[rw + (uint32_t)col] : 0u;
+                if (col < s.dw) pf[ir * G::NC + ic] = rp[col];
             }
         }
         return;
@@ -620,7 +620,9 @@ __device__ __forceinline__ void pass1_tile(const Pass1Args& a, const int bx, con
 
     const int W = (int)a.w, H = (int)a.h;
     const int X0 = bx * tw, Y0 = by * th;
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    // the wave index made wave-uniform: row arithmetic on it (loop bounds, staging rows, tile rows)
+    // then runs on the scalar unit
+    const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     const bool contrast_ok = W >= (int)a.window && H >= (int)a.window;
     const size_t fpx = (size_t)W * H;
     const int r0 = wave * rpw;                        // this wave's output rows [r0, r0 + rpw) of the tile
