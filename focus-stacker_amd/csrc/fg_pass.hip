@@ -614,8 +614,9 @@ __device__ __forceinline__ void pass1_tile(const Pass1Args& a, const int bx, con
     uint64_t* tUw = (uint64_t*)(smem + L.tU);         // [th]
     float* tB = (float*)(smem + L.tB);                // [th][64] hybrid: best weight
     uint16_t* tK = (uint16_t*)(smem + L.tK);          // [th][64] hybrid, frames cached: its slot
-    __shared__ double corners[2][8];                  // region corners in the source of frames k, k+1 (by k & 1)
-    __shared__ double cden[2][4];                     // the homography denominators there
+    __shared__ double corners[8];                     // wave 0: region corners in a frame's source
+    __shared__ double cden[4];                        // the homography denominators there
+    __shared__ StageBox sbox[2];                      // the staging boxes of frames k, k+1 (by k & 1)
     __shared__ int sFull[4];                          // per wave: every region pixel it remapped has full alpha
 
     const int W = (int)a.w, H = (int)a.h;
@@ -636,29 +637,38 @@ __device__ __forceinline__ void pass1_tile(const Pass1Args& a, const int bx, con
     const int gx = X0 - b + lane;                     // region column of this lane
     const bool keep_rgba = a.rgba != nullptr, keep_w = a.weights != nullptr;
 
-    // corner c (threads 0-3) of the region in frame k's source, NaN where not staged
-    auto corner = [&](int k) {
-        if (threadIdx.x >= 4) return;
+    // The staging box of the region in frame k's source, formed by wave 0 alone and published in
+    // slot k & 1: threads 0-3 map the region's corners (NaN where not staged), the wave takes their
+    // bounding box (stage_box), lane 0 stores it.
+    auto stage_next = [&](int k) {
+        if (wave != 0) return;
         // the corner's position is formed anew for every frame: held across the frame loop, it was
         // spilled, and its reload waited (vmcnt) for every store of wave 0 before the barrier
-        int tid = (int)threadIdx.x;
+        int tid = lane;
         asm volatile("" : "+v"(tid));
-        double x = __builtin_nan(""), y = __builtin_nan(""), d = 0.0;
-        if (k < a.k1) {
-            const FrameDesc f = frame_desc(a.frames, k);
-            if (f.kind == 0 && f.p.mode == REMAP_HOMOGRAPHY) {
-                const int cx = X0 - b + ((tid & 1) ? P1C - 1 : 0);
-                const int cy = Y0 - b + ((tid & 2) ? RH - 1 : 0);
-                double sx, sy;
-                if (fast_coord(f.p, col_homog(f.p, cx), cy, &sx, &sy, &d)) {
-                    x = sx;
-                    y = sy;
+        if (tid < 4) {
+            double x = __builtin_nan(""), y = __builtin_nan(""), d = 0.0;
+            if (k < a.k1) {
+                const FrameDesc f = frame_desc(a.frames, k);
+                if (f.kind == 0 && f.p.mode == REMAP_HOMOGRAPHY) {
+                    const int cx = X0 - b + ((tid & 1) ? P1C - 1 : 0);
+                    const int cy = Y0 - b + ((tid & 2) ? RH - 1 : 0);
+                    double sx, sy;
+                    if (fast_coord(f.p, col_homog(f.p, cx), cy, &sx, &sy, &d)) {
+                        x = sx;
+                        y = sy;
+                    }
                 }
             }
+            corners[2 * tid] = x;
+            corners[2 * tid + 1] = y;
+            cden[tid] = d;
         }
-        corners[k & 1][2 * tid] = x;
-        corners[k & 1][2 * tid + 1] = y;
-        cden[k & 1][tid] = d;
+        __builtin_amdgcn_wave_barrier();  // one wave: its LDS accesses take effect in order
+        const StageBox bx = k < a.k1 ? stage_box<T>(corners, cden, frame_desc(a.frames, k))
+                                     : StageBox{0, 0, 0, 0, 0, 0, 0};
+        if (lane == 0) sbox[k & 1] = bx;
+        __builtin_amdgcn_wave_barrier();  // every lane has read the corners before the next frame's
     };
 
 #pragma unroll 1
@@ -689,15 +699,15 @@ __device__ __forceinline__ void pass1_tile(const Pass1Args& a, const int bx, con
 
     // stage the first frame
     uint32_t pf[G::NR * G::NC];
-    corner(a.k0);
-    corner(a.k0 + 1);
+    stage_next(a.k0);
+    stage_next(a.k0 + 1);
     lds_barrier();
     // frames are staged where the region lies inside the stack and its footprint inside the source
     // (StageBox::inside): the staged remap loop then needs no per-pixel checks.  The other tiles
     // (a border of the stack, a source edge) gather their taps from global memory.
     const bool region_in = X0 - b >= 0 && Y0 - b >= 0 && X0 - b + P1C <= W && Y0 - b + RH <= H;
     const FrameDesc f0 = frame_desc(a.frames, a.k0);
-    StageBox box = stage_box<T>(corners[a.k0 & 1], cden[a.k0 & 1], f0);
+    StageBox box = sbox[a.k0 & 1];
     box.ok = box.ok && box.inside && region_in;
     if (box.ok) {
         stage_load<T>(f0, box, lane, wave, pf);
@@ -710,8 +720,7 @@ __device__ __forceinline__ void pass1_tile(const Pass1Args& a, const int bx, con
         // loads are issued before this frame's stores and are in flight through the remap, so
         // waiting for them (before the contrast) does not wait for the contrast's stores
         const FrameDesc fn = frame_desc(a.frames, k + 1 < a.k1 ? k + 1 : k);
-        StageBox nbox = k + 1 < a.k1 ? stage_box<T>(corners[(k + 1) & 1], cden[(k + 1) & 1], fn)
-                                     : StageBox{0, 0, 0, 0, 0, 0, 0};
+        StageBox nbox = sbox[(k + 1) & 1];  // none past the last frame
         nbox.ok = nbox.ok && nbox.inside && region_in;
         if (nbox.ok) stage_load<T>(fn, nbox, lane, wave, pf);
         const FrameDesc f = frame_desc(a.frames, k);
@@ -795,7 +804,7 @@ __device__ __forceinline__ void pass1_tile(const Pass1Args& a, const int bx, con
             const bool wf = __all(lane_full ? 1 : 0);
             if (lane == 0) sFull[wave] = wf ? 1 : 0;
         }
-        corner(k + 2);  // its buffer held frame k's corners, read by every wave before the last barrier
+        stage_next(k + 2);  // its slot held frame k's box, read by every wave before the last barrier
         lds_barrier();
         const bool full = contrast_ok && sFull[0] && sFull[1] && sFull[2] && sFull[3];
         // every wave is done with this frame's staged box: the next one goes in
