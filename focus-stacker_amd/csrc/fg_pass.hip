@@ -192,6 +192,10 @@ struct StageGeom<uint16_t> {
     static constexpr int NR = 10, NC = 2;
 };
 
+// Staged rows sit at a fixed pitch of 64 * NC dwords (the widest box), whatever the box's width:
+// the second tap row of a pixel is then its first plus a constant.
+template <typename T>
+constexpr int kStagePitch = 64 * StageGeom<T>::NC;
 static_assert(kStageBytesU8 == 4 * StageGeom<uint8_t>::NR * 64 * StageGeom<uint8_t>::NC * 4, "staging size");
 static_assert(kStageBytesU16 == 4 * StageGeom<uint16_t>::NR * 64 * StageGeom<uint16_t>::NC * 4, "staging size");
 
@@ -281,7 +285,9 @@ __device__ __forceinline__ StageBox stage_box(const double* cb, const double* cd
     constexpr double m = 1e-6;
     const bool same_sign = (cd[0] > 2e-6 && cd[1] > 2e-6 && cd[2] > 2e-6 && cd[3] > 2e-6) ||
                            (cd[0] < -2e-6 && cd[1] < -2e-6 && cd[2] < -2e-6 && cd[3] < -2e-6);
-    s.inside = same_sign && mnx >= m && mny >= m && mxx <= (double)(sw - 1) - m && mxy <= (double)(sh - 1) - m;
+    // (sources at most 2^24 wide: the staged loop forms x0 * 3 * sizeof(T) by a 24-bit multiply)
+    s.inside = same_sign && mnx >= m && mny >= m && mxx <= (double)(sw - 1) - m && mxy <= (double)(sh - 1) - m &&
+               sw <= (1 << 24);
     return s;
 }
 
@@ -356,7 +362,7 @@ __device__ __forceinline__ void stage_store(const StageBox& s, int lane, int wav
 #pragma unroll
         for (int ic = 0; ic < G::NC; ic++) {
             const int col = lane + 64 * ic;
-            if (row < s.rows && col < s.dw) stage[row * s.dw + col] = pf[ir * G::NC + ic];
+            if (row < s.rows && col < s.dw) stage[row * kStagePitch<T> + col] = pf[ir * G::NC + ic];
         }
     }
 }
@@ -430,27 +436,43 @@ __device__ __forceinline__ bool bilinear_taps_u8(const uint32_t t[12], double fx
     return near ? bilinear_taps(t, fx, fy, band, o) : true;
 }
 
-// u16 interior taps in f64 lerp form on exact integer differences, without the weights' sum and
-// its reciprocal: top = p00 + fx (p10 - p00), bottom likewise, value = top + fy (bottom - top).
-// Against the reference's sum(p w) / sum(w) (weights rounded in f64, sum(w) = 1 to within a few
-// ulp) this differs by a few ulp of 65535, about 1e-10; `band` (4e-6, set by the reciprocal
-// coordinates) is far wider, so outside it both round the same way, and inside it the pixel goes
-// to the exact path.
+// u16 interior taps in f64 on exact integer differences, without the weights' sum and its
+// reciprocal: value = p00 + fx d0 + fy (dd + fx e), d0 = p10 - p00, dd = p01 - p00,
+// e = (p11 - p01) - d0, three fused products.  Against the reference's sum(p w) / sum(w)
+// (weights rounded in f64, sum(w) = 1 to within a few ulp) this differs by a few ulp of 65535,
+// about 1e-10; `band` (4e-6, set by the reciprocal coordinates) is far wider, so outside it both
+// round the same way, and inside it the pixel goes to the exact path.  The tie test is one compare:
+// q = value + (0.5 - bw) with bw = band + 2^-30 (more than the addition's rounding, at most 2^-37
+// below 65536), and where fract(q) < 1 - 2 bw, value + 0.5 lies more than `band` from an integer
+// and truncating q gives its floor, the rounded value.
 __device__ __forceinline__ bool bilinear_taps_lerp64(const uint32_t t[12], double fx, double fy, double band,
                                                      uint32_t o[3]) {
+    const double bw = band + 0x1p-30;
+    const double shift = 0.5 - bw, lim = 1.0 - 2.0 * bw;
     bool ok = true;
 #pragma unroll
     for (int c = 0; c < 3; c++) {
-        const double p00 = (double)t[c], p01 = (double)t[6 + c];
-        const double d0 = (double)((int)t[3 + c] - (int)t[c]), d1 = (double)((int)t[9 + c] - (int)t[6 + c]);
-        const double top = fma(fx, d0, p00);
-        const double bot = fma(fx, d1, p01);
-        const double q = fma(fy, bot - top, top) + 0.5;  // > 0: truncation is the floor
-        const double d = __builtin_amdgcn_fract(q);
-        ok = ok && fabs(d - 0.5) < 0.5 - band;
+        const int p00 = (int)t[c], p10 = (int)t[3 + c], p01 = (int)t[6 + c], p11 = (int)t[9 + c];
+        const int d0 = p10 - p00, dd = p01 - p00, e = (p11 - p01) - d0;
+        const double top = fma(fx, (double)d0, (double)p00);
+        const double in = fma(fx, (double)e, (double)dd);
+        const double q = fma(fy, in, top) + shift;  // > 0: truncation is the floor
+        ok = ok && __builtin_amdgcn_fract(q) < lim;
         o[c] = (uint32_t)q;
     }
     return ok;
+}
+
+// The constant term of a staged tap's byte offset for rows of a multiple of 4 bytes (remap_fast,
+// ALIGNED): the byte of the box's first column within its staged dword, less the offsets of the
+// box's first row and column.  Formed once per frame, and opaque to the compiler, which would
+// otherwise fold its column term back into the per-pixel product and lose the 24-bit multiply.
+template <typename T>
+__device__ __forceinline__ uint32_t aligned_stage_k(const StageBox& s, uint32_t lead) {
+    constexpr uint32_t C3 = 3u * (uint32_t)sizeof(T), PB = 4u * (uint32_t)kStagePitch<T>;
+    uint32_t k = (((lead + (uint32_t)s.x_lo * C3) & 3u) - (uint32_t)s.x_lo * C3) - (uint32_t)s.y_lo * PB;
+    asm volatile("" : "+v"(k));
+    return k;
 }
 
 // Region pixel in row gy of the homography fast path: coordinate by one reciprocal
@@ -458,10 +480,10 @@ __device__ __forceinline__ bool bilinear_taps_lerp64(const uint32_t t[12], doubl
 // staged).  False where the exact path has to decide.  STAGED is the block-uniform choice between
 // the two tap sources: a loop that only reads the staged box holds no global loads, so no
 // iteration waits on the vector-memory counter, which also counts the loop's own RGBA stores.
-template <typename T, bool STAGED, bool INSIDE = false>
+template <typename T, bool STAGED, bool INSIDE = false, bool ALIGNED = false>
 __device__ __forceinline__ bool remap_fast(const FastFrame& ff, const ColHomog& ch, const StageBox& s,
                                            const uint8_t* stage, uint32_t lead, const T* __restrict__ src, int gy,
-                                           double band, uint32_t o[3], double* seed = nullptr) {
+                                           double band, uint32_t o[3], double* seed = nullptr, uint32_t kst = 0) {
     // INSIDE (staged tiles whose whole footprint is in the source interior and the box, StageBox::
     // inside): the checks below hold for every pixel and are skipped
     // the reference's c + (f * (l / denom) + shift) with the focal length folded into the homography
@@ -500,14 +522,25 @@ __device__ __forceinline__ bool remap_fast(const FastFrame& ff, const ColHomog& 
     const double fx = __builtin_amdgcn_fract(x), fy = __builtin_amdgcn_fract(y);
     uint32_t t[12];
     if (STAGED) {
-        if (!INSIDE && (x0 < s.x_lo || y0 < s.y_lo || y0 + 1 >= s.y_lo + s.rows)) return false;
-        const uint32_t xb = (uint32_t)(x0 - s.x_lo) * 3u * (uint32_t)sizeof(T);
-        if (!INSIDE && xb + 6u * (uint32_t)sizeof(T) + 3u > 4u * (uint32_t)s.dw) return false;
-        const uint32_t rowb = ff.sw * 3u * (uint32_t)sizeof(T);
-        const uint32_t g0 = lead + ((uint32_t)y0 * ff.sw + (uint32_t)s.x_lo) * 3u * (uint32_t)sizeof(T);
-        const uint32_t r0 = (uint32_t)(y0 - s.y_lo);
-        const uint32_t a0 = r0 * 4u * (uint32_t)s.dw + (g0 & 3u) + xb;
-        const uint32_t a1 = (r0 + 1u) * 4u * (uint32_t)s.dw + ((g0 + rowb) & 3u) + xb;
+        constexpr uint32_t C3 = 3u * (uint32_t)sizeof(T), PB = 4u * (uint32_t)kStagePitch<T>;
+        uint32_t a0, a1;
+        if constexpr (INSIDE && ALIGNED) {
+            // rows of a multiple of 4 bytes: every staged row starts at the same byte of its first
+            // dword, so the offset is y0 * PB + x0 * C3 + kst, a constant of the box (aligned_stage_k;
+            // modulo 2^32, x0 < 2^24 for the 24-bit multiply, StageBox::inside), and the next row's
+            // is PB further
+            a0 = __umul24((uint32_t)x0, C3) + (uint32_t)y0 * PB + kst;
+            a1 = a0 + PB;
+        } else {
+            if (!INSIDE && (x0 < s.x_lo || y0 < s.y_lo || y0 + 1 >= s.y_lo + s.rows)) return false;
+            const uint32_t xb = (uint32_t)(x0 - s.x_lo) * C3;
+            if (!INSIDE && xb + 6u * (uint32_t)sizeof(T) + 3u > 4u * (uint32_t)s.dw) return false;
+            const uint32_t rowb = ff.sw * C3;
+            const uint32_t g0 = lead + ((uint32_t)y0 * ff.sw + (uint32_t)s.x_lo) * C3;
+            const uint32_t r0 = (uint32_t)(y0 - s.y_lo);
+            a0 = r0 * PB + (g0 & 3u) + xb;
+            a1 = (r0 + 1u) * PB + ((g0 + rowb) & 3u) + xb;
+        }
 #pragma unroll
         for (int c = 0; c < 6; c++) {
             if (sizeof(T) == 1) {
@@ -766,15 +799,25 @@ __device__ __forceinline__ void pass1_tile(const Pass1Args& a, const int bx, con
         // loop, the one call site, which keeps the call out of the loops
         uint32_t redo = 0;
         if (box.ok) {  // staged: no per-pixel checks
-            double seed = 0.0;  // the previous row's reciprocal denominator (none yet)
+            // ALIGNED: source rows of a multiple of 4 bytes, whose staged rows share one alignment
+            auto staged_rows = [&](auto aligned_c) {
+                constexpr bool AL = decltype(aligned_c)::value;
+                const uint32_t kst = AL ? aligned_stage_k<T>(box, lead) : 0u;
+                double seed = 0.0;  // the previous row's reciprocal denominator (none yet)
 #pragma unroll 1
-            for (int r = wave, i = 0; r < RH; r += 4, i++) {
-                const int gy = Y0 - b + r;
-                uint32_t o[4] = {0, 0, 0, SampleMax<T>::i};
-                if (!remap_fast<T, true, true>(ff, chm, box, (const uint8_t*)stage, lead, fsrc, gy, band, o, &seed))
-                    redo |= 1u << i;
-                emit(r, gy, o, std::true_type{});
-            }
+                for (int r = wave, i = 0; r < RH; r += 4, i++) {
+                    const int gy = Y0 - b + r;
+                    uint32_t o[4] = {0, 0, 0, SampleMax<T>::i};
+                    if (!remap_fast<T, true, true, AL>(ff, chm, box, (const uint8_t*)stage, lead, fsrc, gy, band, o,
+                                                       &seed, kst))
+                        redo |= 1u << i;
+                    emit(r, gy, o, std::true_type{});
+                }
+            };
+            if (((ff.sw * 3u * (uint32_t)sizeof(T)) & 3u) == 0)
+                staged_rows(std::true_type{});
+            else
+                staged_rows(std::false_type{});
         } else {
 #pragma unroll 1
             for (int r = wave, i = 0; r < RH; r += 4, i++) {
