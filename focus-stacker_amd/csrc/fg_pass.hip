@@ -415,11 +415,13 @@ __device__ __forceinline__ bool bilinear_taps(const uint32_t t[12], double fx, d
     return ok;
 }
 
-// u8 interior taps in f32, the lerp form on exact integer differences: the value + 0.5 is within
-// 1e-4 of the exact one (weight rounding 2 x 255 x 2^-25, four f32 roundings of at most 2^-17 each
-// at magnitudes below 256, and their propagation).  Outside a band of 2.5e-4 around a rounding tie
-// the exact value, and so the reference's f64 value, rounds the same way; inside it the pixel is
-// redone in f64 by bilinear_taps, whose own band then sends true near-ties to the exact path.
+// u8 interior taps in f32, the lerp form on exact integer differences: the value is within 1e-4 of
+// the exact one (weight rounding 2 x 255 x 2^-25, f32 roundings of at most 2^-17 each at
+// magnitudes below 256, and their propagation), and q = value + (0.5 - 2.5e-4) within 1.16e-4 of
+// exact + 0.5 - 2.5e-4 (one more rounding).  Where fract(q) < 1 - 5e-4, exact + 0.5 lies at least
+// 1.34e-4 from an integer on either side, so the exact value, and so the reference's f64 value,
+// rounds to the truncation of q; elsewhere the pixel is redone in f64 by bilinear_taps, whose own
+// band then sends true near-ties to the exact path.
 __device__ __forceinline__ bool bilinear_taps_u8(const uint32_t t[12], double fx, double fy, double band,
                                                  uint32_t o[3]) {
     const float fx32 = (float)fx, fy32 = (float)fy;
@@ -429,8 +431,8 @@ __device__ __forceinline__ bool bilinear_taps_u8(const uint32_t t[12], double fx
         const float p00 = (float)t[c], p10 = (float)t[3 + c], p01 = (float)t[6 + c], p11 = (float)t[9 + c];
         const float top = fmaf(fx32, p10 - p00, p00);
         const float bot = fmaf(fx32, p11 - p01, p01);
-        const float q = fmaf(fy32, bot - top, top) + 0.5f;  // > 0: truncation is the floor
-        near = near || fabsf(__builtin_amdgcn_fractf(q) - 0.5f) > 0.5f - 2.5e-4f;
+        const float q = fmaf(fy32, bot - top, top) + (0.5f - 2.5e-4f);  // > 0: truncation is the floor
+        near = near || __builtin_amdgcn_fractf(q) >= 1.0f - 5e-4f;
         o[c] = (uint32_t)q;
     }
     return near ? bilinear_taps(t, fx, fy, band, o) : true;
@@ -808,9 +810,13 @@ __device__ __forceinline__ void pass1_tile(const Pass1Args& a, const int bx, con
                 for (int r = wave, i = 0; r < RH; r += 4, i++) {
                     const int gy = Y0 - b + r;
                     uint32_t o[4] = {0, 0, 0, SampleMax<T>::i};
-                    if (!remap_fast<T, true, true, AL>(ff, chm, box, (const uint8_t*)stage, lead, fsrc, gy, band, o,
-                                                       &seed, kst))
-                        redo |= 1u << i;
+                    const bool ok = remap_fast<T, true, true, AL>(ff, chm, box, (const uint8_t*)stage, lead, fsrc, gy,
+                                                                  band, o, &seed, kst);
+                    // near-ties are rare: the mask is updated behind a wave vote
+                    if (__builtin_amdgcn_ballot_w64(!ok)) {
+                        asm volatile("" ::: "memory");
+                        if (!ok) redo |= 1u << i;
+                    }
                     emit(r, gy, o, std::true_type{});
                 }
             };
@@ -1308,7 +1314,9 @@ __device__ __forceinline__ void pass2_tile(const Pass2Args& a, const LevelArgs& 
     __shared__ __align__(16) float rG[NREG];
     __shared__ __align__(16) float rB[NREG];
     __shared__ __align__(16) float rM[NREG];
-    __shared__ float2 sSY[L0 ? NREG : 1];      // norm sum (-1 where no frame has support), RN(1 / norm sum)
+    // norm sum and RN(1 / norm sum) where some frame has support and the sum exceeds 1e-12; elsewhere
+    // -1 and the pixel's mask value instead (1/N with support, -1 without)
+    __shared__ float2 sSY[L0 ? NREG : 1];
     __shared__ float nR[NH * NW], nG[NH * NW], nB[NH * NW];   // level-(l+1) gaussian (values)
     __shared__ float hR[NH * TW], hG[NH * TW], hB[NH * TW];   // its horizontal expansion per tile column
     __shared__ int sPart[2];                   // per frame parity: some region pixel lacks support
@@ -1344,12 +1352,16 @@ __device__ __forceinline__ void pass2_tile(const Pass2Args& a, const LevelArgs& 
 #pragma unroll 1
         for (int idx = threadIdx.x; idx < NREG; idx += 256) {
             const int gx = X0 - 4 + idx % RW, gy = Y0 - 4 + idx / RW;
-            float v = -1.0f, y = 0.0f;
+            float v = -1.0f, y = -1.0f;
             if (gx >= 0 && gy >= 0 && gx < g.w && gy < g.h) {
                 const int i = gy * g.w + gx;
                 if (a.union_sup[i]) {
                     v = a.norm_sum[i];
                     y = 1.0f / v;
+                    if (!(v > 1e-12f)) {
+                        v = -1.0f;
+                        y = m_unsup;
+                    }
                 }
             }
             sSY[idx] = make_float2(v, y);
@@ -1410,7 +1422,8 @@ __device__ __forceinline__ void pass2_tile(const Pass2Args& a, const LevelArgs& 
                         v.y = (float)Pack<T>::ch(q, 1);
                         v.z = (float)Pack<T>::ch(q, 2);
                         // w / S by the Markstein correction from RN(1/S) (div_by_recip), formed
-                        // without branches; 1/N where S <= 1e-12, -1 without support
+                        // without branches; where the norm sum is marked (-1), the mask value held
+                        // in place of its reciprocal (1/N where S <= 1e-12, -1 without support)
                         const float2 sy = sSY[idx];
                         const float sv = sy.x, w = pf_w[j];
                         const float qd = w * sy.y;
@@ -1422,8 +1435,7 @@ __device__ __forceinline__ void pass2_tile(const Pass2Args& a, const LevelArgs& 
                             asm volatile("" ::: "memory");  // a real branch: the division is not speculated
                             if (tiny) mw = w / sv;
                         }
-                        mw = sv > 1e-12f ? mw : m_unsup;
-                        v.w = sv < 0.0f ? -1.0f : mw;
+                        v.w = sv > 0.0f ? mw : sy.y;
                     } else {
                         v = pf_v[j];
                     }
