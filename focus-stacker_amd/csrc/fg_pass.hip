@@ -1070,11 +1070,69 @@ static hipError_t launch_pass1_k(const Pass1Args& a, hipStream_t s) {
     return hipGetLastError();
 }
 
+// Hybrid winners of frames [k0, k1) from their cached weights (updateRange, src/fuse/blend.zig:
+// 237-266): per pixel the strictly greater weight wins, so the first frame wins ties; `best` carries
+// the running maximum across launches, and a new winner's colour comes from the frame cache.  The
+// weights are the ones pass 1 stored, so the winners are those pass 1 would keep; kept apart, the
+// pass-1 launch carries no winner state in LDS and runs four blocks per CU.
+template <typename T>
+__global__ void __launch_bounds__(256) k_hybrid_winners(Pass1Args a) {
+    const size_t npx = (size_t)a.w * a.h;
+    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= npx) return;
+    const float* wp = a.weights + (size_t)(a.k0 - a.slot_base) * npx + i;
+    const int n = a.k1 - a.k0;
+    float b = a.best[i];
+    int win = -1;
+    int k = 0;
+    for (; k + 8 <= n; k += 8) {  // eight loads in flight
+        float w[8];
+#pragma unroll
+        for (int j = 0; j < 8; j++) w[j] = wp[(size_t)(k + j) * npx];
+#pragma unroll
+        for (int j = 0; j < 8; j++) {
+            if (w[j] > b) {
+                b = w[j];
+                win = k + j;
+            }
+        }
+    }
+    for (; k < n; k++) {
+        const float w = wp[(size_t)k * npx];
+        if (w > b) {
+            b = w;
+            win = k;
+        }
+    }
+    if (win >= 0) {
+        a.best[i] = b;
+        const T* px = (const T*)a.rgba + ((size_t)(a.k0 - a.slot_base + win) * npx + i) * 4;
+        T* hd = (T*)a.hard + i * 3;
+        hd[0] = px[0];
+        hd[1] = px[1];
+        hd[2] = px[2];
+    }
+}
+
 template <typename T, int RH>
 static hipError_t launch_pass1_t(const Pass1Args& a, hipStream_t s) {
     const bool slots = pass1_slot_winners(a);
     if (a.best != nullptr && !slots) return launch_pass1_k<T, RH, true, false>(a, s);
-    if constexpr (sizeof(T) == 1) {
+    if constexpr (sizeof(T) == 2) {
+        if (a.best != nullptr && a.norm_sum != nullptr && a.weights != nullptr) {
+            // cached 16-bit hybrid: the plain pass 1 (four blocks per CU, where the winners' LDS
+            // left three), then the winners from the weights it stored.  The 8-bit hybrid keeps
+            // them in the packed pass-1 layout, which is faster there.
+            Pass1Args p = a;
+            p.best = nullptr;
+            p.hard = nullptr;
+            hipError_t e = launch_pass1_k<T, RH, false, false>(p, s);
+            if (e != hipSuccess) return e;
+            const size_t npx = (size_t)a.w * a.h;
+            hipLaunchKernelGGL(k_hybrid_winners<T>, dim3((unsigned)((npx + 255) / 256)), dim3(256), 0, s, a);
+            return hipGetLastError();
+        }
+    } else {
         if (a.best != nullptr && a.norm_sum != nullptr) return launch_pass1_k<T, RH, false, true>(a, s);
     }
     return launch_pass1_k<T, RH, false, false>(a, s);

@@ -447,12 +447,13 @@ def test_graph_replays_equal_eager(ctx, fg, method):
     s.close()
 
 
-def test_hybrid_host_frames_several_launches(ctx, fg, oracle):
+@pytest.mark.parametrize("dtype", ["u8", "u16"])
+def test_hybrid_host_frames_several_launches(ctx, fg, oracle, dtype):
     """Hybrid fed from host buffers: pass 1 runs once per 8 uploaded frames, and each launch's
-    winners (kept as frame slots of the cache) are resolved against the hard image of the earlier
-    launches."""
+    winners are resolved against the hard image of the earlier launches (u8: kept as frame slots of
+    the cache inside pass 1; u16: taken from the cached weights by the winners kernel after it)."""
     from oracle import make_pose, METHOD_HYBRID
-    spec, frames, poses = _synthetic(dict(frames=11, width=160, height=120))
+    spec, frames, poses = _synthetic(dict(frames=11, width=160, height=120, dtype=dtype))
     opo = [make_pose(**p) for p in poses]
     roi = oracle.common_overlap_roi([(spec.width, spec.height)] * spec.frames, opo)
     s, out = _run_stack(fg, ctx, frames, poses, roi, method=fg.METHOD_HYBRID, debug_frame=1)
