@@ -415,19 +415,20 @@ def test_unaligned_device_frames(ctx, fg, dtype):
     assert np.array_equal(out, ref)
 
 
-@pytest.mark.parametrize("method", ["pyramid", "hybrid"])
-def test_graph_replays_equal_eager(ctx, fg, method):
+@pytest.mark.parametrize("method,dtype", [("pyramid", "u8"), ("hybrid", "u8"), ("hybrid", "u16")])
+def test_graph_replays_equal_eager(ctx, fg, method, dtype):
     """A cached stack finished again runs pass 2 + collapse as a HIP graph captured on its second
     finish; with profiling on it launches the same kernels eagerly.  Two frame sets alternate through one stack
     (eager pass 1, reset and synchronisation between the runs), so a replay that kept stale data
-    or a captured clear that misbehaved on later replays would show."""
+    or a captured clear that misbehaved on later replays would show.  The u16 hybrid takes its
+    winners from the cached weights after pass 1."""
     from oracle import make_pose
     m = fg.METHOD_PYRAMID if method == "pyramid" else fg.METHOD_HYBRID
-    spec, fa, poses = _synthetic(dict(frames=4, width=160, height=120))
+    spec, fa, poses = _synthetic(dict(frames=4, width=160, height=120, dtype=dtype))
     fb = [np.ascontiguousarray(f[:, ::-1]) for f in fa[::-1]]   # a different stack of the same geometry
     roi = fg.common_overlap_roi([(spec.width, spec.height)] * 4, [fg.pose(p) for p in poses])
     w, h = roi[2] - roi[0], roi[3] - roi[1]
-    s = fg.Stack(ctx, 4, w, h, method=m)
+    s = fg.Stack(ctx, 4, w, h, fg.SAMPLE_U8 if dtype == "u8" else fg.SAMPLE_U16, method=m)
 
     def run(frames):
         s.reset()
