@@ -137,7 +137,7 @@ static int images_close(const void* a, const void* b, size_t n, int bytes, int* 
     return mx <= 1 && *meand < 0.05;
 }
 
-/* hardmask: winners flip only where two frames' weights tie to the last ulp */
+/* share of the pixels whose rgb differs between two images (the hardmask winners must all agree) */
 static double pixels_differing(const void* a, const void* b, size_t npx3, int bytes) {
     size_t n = 0;
     for (size_t p = 0; p < npx3 / 3; p++) {
