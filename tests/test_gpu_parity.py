@@ -578,7 +578,7 @@ def _sweep_case(seed):
                 roll=float(rng.uniform(0.0, 0.3)), scale=float(rng.uniform(0.0, 0.01)))
 
 
-@pytest.mark.parametrize("seed", range(48))
+@pytest.mark.parametrize("seed", range(256))
 def test_random_stacks_match_oracle(ctx, fg, oracle, seed):
     """Seeded sweep over whole stacks through the C ABI, each against the oracle with the same
     options; the case is printed first so a failure names it."""
@@ -595,7 +595,7 @@ def test_random_stacks_match_oracle(ctx, fg, oracle, seed):
     from oracle import make_pose
     opo = [make_pose(**p) for p in poses]
     roi = oracle.common_overlap_roi([(spec.width, spec.height)] * spec.frames, opo)
-    if roi[2] - roi[0] < 1 or roi[3] - roi[1] < 1:
+    if roi is None or roi[2] - roi[0] < 1 or roi[3] - roi[1] < 1:  # tiny frames shifted apart
         roi = (0, 0, spec.width, spec.height)
     fm = {"pyramid": fg.METHOD_PYRAMID, "hybrid": fg.METHOD_HYBRID, "hardmask": fg.METHOD_HARDMASK,
           "softmask": fg.METHOD_SOFTMASK}[c["method"]]
