@@ -6,6 +6,7 @@
 #include <math.h>
 
 #include "fg_internal.h"
+#include "fg_seqsum.h"
 
 namespace fg {
 
@@ -81,7 +82,15 @@ __device__ __forceinline__ void invert_radial(const RemapParams& p, double dx, d
 // Source coordinate of output pixel (x, y).  Returns false when the reference writes zeros
 // without sampling (|denom| < 1e-12 on the lens path, src/remap.zig:420-422; rays behind
 // the camera on the basic path, whose legacy chain maps them 1.6e16 px away).
-__device__ __forceinline__ bool map_pixel(const RemapParams& p, int x, int y, double* sx, double* sy) {
+//
+// The lens path: the reference walks each output row from the ROI's left edge, adding the
+// homography's first column once per pixel (remapU8RowsNoTranslationLens / remapU16RowsNoTranslationLens,
+// src/remap.zig:397-437, 497-537), so pixel x sees x roundings.  WALK forms the rows that way
+// (seq_add); otherwise they are the products B (x - cx, y - cy, -d), and `err` (when given) receives
+// a bound on |sx - sx_walk| + |sy - sy_walk| in source pixels.
+template <bool WALK = false>
+__device__ __forceinline__ bool map_pixel(const RemapParams& p, int x, int y, double* sx, double* sy,
+                                          double* err = nullptr) {
     double px = (double)(p.roi_left + x) - p.dcx;
     double py = (double)(p.roi_top + y) - p.dcy;
     double rz = p.neg_d;
@@ -98,9 +107,17 @@ __device__ __forceinline__ bool map_pixel(const RemapParams& p, int x, int y, do
         py = hy - p.ty;
         rz = hz - p.tz;
     }
-    double lx = p.B[0] * px + p.B[1] * py + p.B[2] * rz;
-    double ly = p.B[3] * px + p.B[4] * py + p.B[5] * rz;
-    double lz = p.B[6] * px + p.B[7] * py + p.B[8] * rz;
+    double lx, ly, lz;
+    if (WALK && p.mode == REMAP_LENS) {
+        const double bx = (double)p.roi_left - p.dcx;
+        lx = seq_add(p.B[0] * bx + p.B[1] * py + p.B[2] * rz, p.B[0], x);
+        ly = seq_add(p.B[3] * bx + p.B[4] * py + p.B[5] * rz, p.B[3], x);
+        lz = seq_add(p.B[6] * bx + p.B[7] * py + p.B[8] * rz, p.B[6], x);
+    } else {
+        lx = p.B[0] * px + p.B[1] * py + p.B[2] * rz;
+        ly = p.B[3] * px + p.B[4] * py + p.B[5] * rz;
+        lz = p.B[6] * px + p.B[7] * py + p.B[8] * rz;
+    }
     double denom = -lz;
     double rx, ry;
     if (p.mode == REMAP_TRANSLATION) {
@@ -120,6 +137,25 @@ __device__ __forceinline__ bool map_pixel(const RemapParams& p, int x, int y, do
     if (p.legacy ? !(denom > 0.0) : fabs(denom) < 1e-12) return false;
     rx = p.focal * (lx / denom) + p.csx;
     ry = p.focal * (ly / denom) + p.csy;
+    if (!WALK && err && p.mode == REMAP_LENS) {
+        // each walked row value differs from its product form by at most half an ulp of the row's
+        // largest |value| per addition (x of them: the values are monotone along the row, so the
+        // largest sits at an end) plus a few ulp of the terms of the start's and the products' sums
+        const double apx = fabs(px), apy = fabs(py), arz = fabs(rz);
+        auto dl = [&](double l, const double* b) {
+            double m = fmax(fabs(l), fabs(l - (double)x * b[0]));
+            m = fmax(m, fmax(fabs(b[0]) * (apx + (double)x), fmax(fabs(b[1]) * apy, fabs(b[2]) * arz)));
+            return ((double)x * 0.5 + 8.0) * ldexp(1.0, ilogb(fmax(m, 1e-300)) - 52);
+        };
+        const double dlx = dl(lx, &p.B[0]), dly = dl(ly, &p.B[3]), dlz = dl(lz, &p.B[6]);
+        const double ad = fabs(denom);
+        const double drx = p.focal * (dlx + fabs(lx / denom) * dlz) / ad;
+        const double dry = p.focal * (dly + fabs(ly / denom) * dlz) / ad;
+        // the radial inverse moves a point by at most about its own displacement (2x for the lens
+        // models' derivative) and stops its Newton steps within 1e-9 px, so one more or one fewer
+        // step moves it by that much
+        *err = 2.0 * (drx + dry) + 2e-9;
+    }
     if (p.mode == REMAP_LENS && p.radial) {
         double ux, uy;
         invert_radial(p, rx, ry, &ux, &uy);
@@ -151,13 +187,24 @@ __device__ __forceinline__ const __attribute__((address_space(1))) T* as_global(
 }
 
 // samplePixelBilinear{U8,U16} with interior and support-renormalised border cases.
+//
+// `band` >= 0: false when a rounded quantity (a sample's quotient, or the border alpha) lies
+// within `band` of its rounding tie (for the border quotients v / sup, 2 band / sup: v and sup both
+// move with the coordinate), or the border support is below 1/4.
 template <typename T, int CH>
-__device__ __forceinline__ void sample_bilinear(const T* __restrict__ src, int w, int h, double x, double y,
-                                                uint32_t out[CH + 1]) {
+__device__ __forceinline__ bool sample_bilinear(const T* __restrict__ src, int w, int h, double x, double y,
+                                                uint32_t out[CH + 1], double band = -1.0) {
     const __attribute__((address_space(1))) T* px = as_global(src);
 #pragma unroll
     for (int c = 0; c <= CH; c++) out[c] = 0;
-    if (!(x >= -1.0 && x < (double)w && y >= -1.0 && y < (double)h)) return;
+    // a coordinate within band of the sampled area's edge may fall on either side of it
+    if (!(x >= -1.0 && x < (double)w && y >= -1.0 && y < (double)h))
+        return !(band >= 0.0 && x >= -1.0 - band && x < (double)w + band && y >= -1.0 - band && y < (double)h + band);
+    bool ok = true;
+    auto decided = [&](double q, double bd) {
+        const double f = q - floor(q);
+        ok = ok && !(fabs(f - 0.5) <= bd);
+    };
     const int x0 = (int)floor(x), y0 = (int)floor(y);
     const int x1 = x0 + 1, y1 = y0 + 1;
     const double fx = x - (double)x0, fy = y - (double)y0;
@@ -174,9 +221,10 @@ __device__ __forceinline__ void sample_bilinear(const T* __restrict__ src, int w
             const double v = (double)px[b00 + c] * w00 + (double)px[b10 + c] * w10 + (double)px[b01 + c] * w01 +
                              (double)px[b11 + c] * w11;
             out[c] = round_div(v, sup, inv);
+            if (band >= 0.0) decided(v * inv, band);
         }
         out[CH] = SampleMax<T>::i;
-        return;
+        return ok;
     }
     const bool i00 = x0 >= 0 && y0 >= 0 && x0 < w && y0 < h;
     const bool i10 = x1 >= 0 && y0 >= 0 && x1 < w && y0 < h;
@@ -187,7 +235,7 @@ __device__ __forceinline__ void sample_bilinear(const T* __restrict__ src, int w
     if (i10) sup += w10;
     if (i01) sup += w01;
     if (i11) sup += w11;
-    if (sup <= 0.0) return;
+    if (sup <= 0.0) return !(band >= 0.0);
     const double inv = 1.0 / sup;
 #pragma unroll
     for (int c = 0; c < CH; c++) {
@@ -197,8 +245,42 @@ __device__ __forceinline__ void sample_bilinear(const T* __restrict__ src, int w
         if (i01) v += (double)px[(y1 * w + x0) * CH + c] * w01;
         if (i11) v += (double)px[(y1 * w + x1) * CH + c] * w11;
         out[c] = round_div(v, sup, inv);
+        if (band >= 0.0) decided(v * inv, 2.0 * band / sup);
     }
     out[CH] = (uint32_t)round(sup * SampleMax<T>::d);
+    if (band >= 0.0) {
+        decided(sup * SampleMax<T>::d, band);
+        ok = ok && sup >= 0.25;
+    }
+    return ok;
+}
+
+// remapRigidImage's sample for output pixel (x, y): RGB(A) into o.  The lens path takes its row
+// values as products, and walks the row (map_pixel<true>) only where a rounding tie lies within
+// reach of their difference: a quotient moves by at most the sample range times the coordinate
+// change (|dv/dx|, |dv/dy| <= max over the taps' differences, the support weights summing to 1),
+// and so does the border alpha.
+template <typename T, int CH>
+__device__ __forceinline__ void remap_sample(const RemapParams& p, const T* __restrict__ src, int sw, int sh, int x,
+                                             int y, uint32_t o[CH + 1]) {
+    double sx, sy, err = 0.0;
+    if (map_pixel(p, x, y, &sx, &sy, &err)) {
+        if (p.mode != REMAP_LENS) {
+            sample_bilinear<T, CH>(src, sw, sh, sx, sy, o);
+            return;
+        }
+        if (sample_bilinear<T, CH>(src, sw, sh, sx, sy, o, 1.25 * SampleMax<T>::d * err + 1e-9)) return;
+    } else if (p.mode != REMAP_LENS) {
+#pragma unroll
+        for (int c = 0; c <= CH; c++) o[c] = 0;
+        return;
+    }
+    if (map_pixel<true>(p, x, y, &sx, &sy)) {
+        sample_bilinear<T, CH>(src, sw, sh, sx, sy, o);
+    } else {
+#pragma unroll
+        for (int c = 0; c <= CH; c++) o[c] = 0;
+    }
 }
 
 // EXPAND of level (sw, sh) to (dw, dh) at output (x, y): expandRgb src/fuse/pyramid.zig:849-909.

@@ -9,8 +9,8 @@ namespace fg {
 
 // How an output (panorama) pixel maps to a source pixel.
 enum RemapMode : int32_t {
-    REMAP_HOMOGRAPHY = 0,   // basic rectilinear (legacy chain == homography) or lens path without radial terms
-    REMAP_LENS = 1,         // lens path with radial terms: homography + Newton undistort
+    REMAP_HOMOGRAPHY = 0,   // basic rectilinear: the legacy chain, evaluated as its homography
+    REMAP_LENS = 1,         // the lens row walk: homography (+ Newton undistort with radial terms)
     REMAP_TRANSLATION = 2,  // translation-plane path
 };
 

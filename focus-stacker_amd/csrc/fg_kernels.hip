@@ -20,13 +20,7 @@ __global__ void __launch_bounds__(256) k_remap(const T* __restrict__ src, int sw
     int y = blockIdx.y * 4 + (threadIdx.x >> 6);
     if (x >= ow || y >= oh) return;
     uint32_t o[CH + 1];
-    double sx, sy;
-    if (map_pixel(p, x, y, &sx, &sy)) {
-        sample_bilinear<T, CH>(src, sw, sh, sx, sy, o);
-    } else {
-#pragma unroll
-        for (int c = 0; c <= CH; c++) o[c] = 0;
-    }
+    remap_sample<T, CH>(p, src, sw, sh, x, y, o);
     T* d = dst + ((size_t)y * ow + x) * (CH + 1);
 #pragma unroll
     for (int c = 0; c <= CH; c++) d[c] = (T)o[c];

@@ -54,12 +54,7 @@ struct Pack<uint16_t> {
 template <typename T>
 __device__ __forceinline__ void frame_rgba(const FrameDesc& f, int x, int y, uint32_t o[4]) {
     if (f.kind == 0) {
-        double sx, sy;
-        if (map_pixel(f.p, x, y, &sx, &sy)) {
-            sample_bilinear<T, 3>((const T*)f.src, (int)f.sw, (int)f.sh, sx, sy, o);
-        } else {
-            o[0] = o[1] = o[2] = o[3] = 0;
-        }
+        remap_sample<T, 3>(f.p, (const T*)f.src, (int)f.sw, (int)f.sh, x, y, o);
     } else {
         const int st = 3 + f.extra;
         const T* q = (const T*)f.src + ((size_t)y * f.sw + x) * st;

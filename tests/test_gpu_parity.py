@@ -160,6 +160,34 @@ def test_remap_real_solved_pose(ctx, fg, oracle):
     _remap_close(got, ref)
 
 
+LENS_ROW_WALK = {
+    # medium-size u16 lens stacks of the seeded generator (stack, frame): one sample of each rounded
+    # the other way while the kernels formed the homography rows as products instead of walking them
+    "1425x1259": (dict(frames=8, width=1425, height=1259, dtype="u16", lens=True, shift=3.474306049699964,
+                       roll_deg=0.22336276936148092, scale=0.0007416607568523637, sigma_max=3.0), 1340, 7),
+    "1618x892": (dict(frames=8, width=1618, height=892, dtype="u16", lens=True, shift=0.11441781908806714,
+                      roll_deg=0.037218691316370164, scale=0.005976056069915496, sigma_max=3.0), 1341, 2),
+}
+
+
+@pytest.mark.parametrize("case", list(LENS_ROW_WALK))
+def test_lens_row_walk_remap_exact(ctx, fg, oracle, case):
+    """The lens path walks each output row adding the homography's first column once per pixel
+    (src/remap.zig:397-437), so pixel x sees x roundings; the remap reproduces that sum exactly
+    (fg_seqsum.h), and every sample equals the oracle's, which walks the row the same way."""
+    import synth
+    from oracle import make_pose
+    kw, stack_id, k = LENS_ROW_WALK[case]
+    spec = synth.config_spec(0, stack_id=stack_id, **kw)
+    st = synth.Stack(spec)
+    f = st.frame_numpy(k)
+    opo = [make_pose(**p) for p in st.poses]
+    roi = oracle.common_overlap_roi([(spec.width, spec.height)] * spec.frames, opo)
+    got = ctx.remap(f, fg.pose(st.poses[k]), roi)
+    ref = oracle.remap(f, opo[k], roi)
+    assert np.array_equal(got, ref), f"{int((got != ref).sum())} samples differ"
+
+
 # --------------------------------------------------------------------------
 # whole stacks
 # --------------------------------------------------------------------------
