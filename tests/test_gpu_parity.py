@@ -373,7 +373,7 @@ def test_cpu_runnable_config_matches_oracle(ctx, fg, oracle, levels):
     ref, dbg = oracle.stack(frames, opo, roi, max_levels=levels, debug=True, frame_index=3)
     assert len(s.levels()) == (11 if levels == 0 else levels)
     d = _img_close(out, ref)
-    print(f"cfg0 levels={levels} roi={roi}: max {d.max()} mean {d.mean():.2e}")
+    print(f"8 x 1024x768 u8 levels={levels} roi={roi}: max {d.max()} mean {d.mean():.2e}")
     _level_close(s.readback("norm_sum"), dbg["norm_sum"], 1e-5)
     for l in range(len(s.levels())):
         _level_close(s.readback("acc_level", l), dbg["acc_levels"][l], 1e-5)
