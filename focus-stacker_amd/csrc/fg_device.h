@@ -54,11 +54,16 @@ __device__ __forceinline__ int clamp_coord(int c, int limit) {  // clampCoord sr
 
 // Newton inverse of the radial model (invertRadialDistortionCached src/optimize.zig:4204-4235,
 // undistortSourcePoint src/remap.zig:755-786); returns the undistorted offset.
-__device__ __forceinline__ void invert_radial(const RemapParams& p, double dx, double dy, double* ux, double* uy) {
+// `sens` (when given): how far the undistorted point moves per pixel that (dx, dy) moves, to first
+// order: the larger of the radial derivative 1 / (d (r factor(r)) / dr) and the tangential scale
+// radius / q at the solution.
+__device__ __forceinline__ void invert_radial(const RemapParams& p, double dx, double dy, double* ux, double* uy,
+                                              double* sens = nullptr) {
     double q = sqrt(dx * dx + dy * dy);
     if (q < 1e-12) {
         *ux = dx;
         *uy = dy;
+        if (sens) *sens = 1.0;
         return;
     }
     double radius = q;
@@ -77,6 +82,13 @@ __device__ __forceinline__ void invert_radial(const RemapParams& p, double dx, d
     double scale = radius / q;
     *ux = dx * scale;
     *uy = dy * scale;
+    if (sens) {
+        const double nr2 = radius * radius * p.inv_norm, nr4 = nr2 * nr2;
+        const double factor = 1.0 + p.ra * nr2 + p.rb * nr4 + p.rc * nr4 * nr2;
+        const double fd = p.ra + 2.0 * p.rb * nr2 + 3.0 * p.rc * nr4;
+        const double df = fabs(factor + 2.0 * radius * radius * p.inv_norm * fd);
+        *sens = fmax(scale, df > 1e-3 ? 1.0 / df : 1e3);
+    }
 }
 
 // Source coordinate of output pixel (x, y).  Returns false when the reference writes zeros
@@ -137,6 +149,7 @@ __device__ __forceinline__ bool map_pixel(const RemapParams& p, int x, int y, do
     if (p.legacy ? !(denom > 0.0) : fabs(denom) < 1e-12) return false;
     rx = p.focal * (lx / denom) + p.csx;
     ry = p.focal * (ly / denom) + p.csy;
+    double dr = 0.0;  // bound on |rx - rx_walk| + |ry - ry_walk|
     if (!WALK && err && p.mode == REMAP_LENS) {
         // each walked row value differs from its product form by at most half an ulp of the row's
         // largest |value| per addition (x of them: the values are monotone along the row, so the
@@ -149,21 +162,23 @@ __device__ __forceinline__ bool map_pixel(const RemapParams& p, int x, int y, do
         };
         const double dlx = dl(lx, &p.B[0]), dly = dl(ly, &p.B[3]), dlz = dl(lz, &p.B[6]);
         const double ad = fabs(denom);
-        const double drx = p.focal * (dlx + fabs(lx / denom) * dlz) / ad;
-        const double dry = p.focal * (dly + fabs(ly / denom) * dlz) / ad;
-        // the radial inverse moves a point by at most about its own displacement (2x for the lens
-        // models' derivative) and stops its Newton steps within 1e-9 px, so one more or one fewer
-        // step moves it by that much
-        *err = 2.0 * (drx + dry) + 2e-9;
+        dr = p.focal * (dlx + fabs(lx / denom) * dlz) / ad + p.focal * (dly + fabs(ly / denom) * dlz) / ad;
     }
+    double sens = 1.0;
     if (p.mode == REMAP_LENS && p.radial) {
         double ux, uy;
-        invert_radial(p, rx, ry, &ux, &uy);
+        invert_radial(p, rx, ry, &ux, &uy, !WALK && err ? &sens : nullptr);
         *sx = p.scx + ux;
         *sy = p.scy + uy;
     } else {
         *sx = p.scx + rx;
         *sy = p.scy + ry;
+    }
+    if (!WALK && err && p.mode == REMAP_LENS) {
+        // the radial inverse scales the difference by its local sensitivity (first order; 1.25 for
+        // what that leaves out) and stops its Newton steps within 1e-9 of the radius equation, so
+        // one more or one fewer step moves the point by up to 1e-9 sens on each run
+        *err = 1.25 * sens * dr + 2e-9 * fmax(sens, 1.0);
     }
     return true;
 }
