@@ -463,6 +463,7 @@ int flush_pass1(fg_stack* s) {
     a.union_sup = s->U;
     a.best = s->best;
     a.hard = s->hard;
+    a.lens = fg::frames_use_lens(s->fdesc.data() + k0, k1 - k0) ? 1 : 0;
     const bool softmask = s->opts.method == FG_METHOD_SOFTMASK_CONTRAST;
     if (s->single) {
         a.slot_base = (int)k0;
@@ -970,6 +971,7 @@ static int pass2_frame(fg_stack* s, const fg_image_desc* d, const void* px, bool
     a.window = s->opts.contrast_window_size;
     a.rgba = s->frames;
     a.weights = s->weights;
+    a.lens = fg::frames_use_lens(&s->fdesc[k], 1) ? 1 : 0;
     {
         PhaseMark pm(s, FG_PHASE_PASS1);
         HIP_TRY(fg::launch_pass1(a, s->stream), "pass1 kernel");

@@ -86,6 +86,8 @@ struct Plan {
     std::vector<CollapseLaunch> collapses;
     std::vector<int> single_level;         // stacks of one level: quantized directly
     hipGraphExec_t graph = nullptr;
+    bool lens = false;                     // this run's frames include lens row-walk remaps
+    bool graph_lens = false;               // ... as they did when the graph was captured
     bool eager = false;                    // capture failed once: launch one by one
     int runs = 0;                          // runs so far: the schedule is captured on the second
     hipStream_t stream = nullptr;          // the stream of its runs (a plan runs on one stream only)
@@ -293,8 +295,8 @@ int enqueue_schedule(Plan* p) {
     hipStream_t s = p->stream;
     if (fg::launch_zero_batch(p->zjobs.p, p->zitems.p, (int)p->zitems.n, s) != hipSuccess)
         return bfail(FG_ERR_DEVICE, "batch clear launch");
-    if (fg::launch_pass1_batch(p->sb, p->window, p->p1.p, p->items1.p, (int)p->items1.n, p->queue, p->blocks1, s) !=
-        hipSuccess)
+    if (fg::launch_pass1_batch(p->sb, p->window, p->lens, p->p1.p, p->items1.p, (int)p->items1.n, p->queue,
+                               p->blocks1, s) != hipSuccess)
         return bfail(FG_ERR_DEVICE, "batch pass-1 launch");
     for (const LevelLaunch& ll : p->levels)
         if (fg::launch_pass2_batch(p->sb, ll.level0, ll.tw, ll.th, ll.jobs.p, ll.items.p, (int)ll.items.n, s) !=
@@ -331,6 +333,13 @@ int run_plan(Plan* p, const std::vector<const fg_request*>& reqs) {
         hipEventRecord(p->staged, stream) != hipSuccess)
         return bfail(FG_ERR_DEVICE, "batch frame upload");
     p->staged_pending = true;
+    // the pass-1 kernel carries the lens-path remap only for runs that have such frames: a graph
+    // captured with the other kernel is captured again
+    p->lens = fg::frames_use_lens(p->staging, p->frames.n);
+    if (p->graph && p->graph_lens != p->lens) {
+        (void)hipGraphExecDestroy(p->graph);
+        p->graph = nullptr;
+    }
     // a geometry's first run launches its schedule one by one (a batch that never recurs pays for
     // no capture); the second captures it, and it is replayed from then on
     if (!p->graph && !p->eager && p->runs > 0) {
@@ -342,6 +351,7 @@ int run_plan(Plan* p, const std::vector<const fg_request*>& reqs) {
             e = hipStreamEndCapture(stream, &g);
         }
         if (rc == FG_OK && e == hipSuccess && g) e = hipGraphInstantiate(&p->graph, g, nullptr, nullptr, 0);
+        p->graph_lens = p->lens;
         if (g) (void)hipGraphDestroy(g);
         if (rc != FG_OK || e != hipSuccess || !p->graph) {
             (void)hipGetLastError();

@@ -63,6 +63,11 @@ struct FrameDesc {
     int32_t extra;     // aligned frame carries an alpha channel
     RemapParams p;
 };
+inline bool frames_use_lens(const FrameDesc* f, size_t n) {
+    for (size_t i = 0; i < n; i++)
+        if (f[i].kind == 0 && f[i].p.mode == REMAP_LENS) return true;
+    return false;
+}
 
 // Pass 1 over frames [k0, k1): remap (or load) each frame into the RGBA cache, contrast
 // weight * alpha into the weight cache, and norm sum / union support / hybrid winners.
@@ -78,6 +83,7 @@ struct Pass1Args {
     uint8_t* union_sup;
     float* best;              // hybrid only, else nullptr
     void* hard;
+    int lens;                 // some frame of [k0, k1) is remapped by the lens row walk (REMAP_LENS)
 };
 hipError_t launch_pass1(const Pass1Args& a, hipStream_t s);
 
@@ -155,8 +161,9 @@ size_t pass1_batch_lds(int sample_bytes, uint32_t window);
 int pass1_resident_blocks(int sample_bytes, int device);
 // Pass 1 (pyramid-contrast, frames cached) of every stack in `stacks` over the tiles in `items`,
 // pulled by `blocks` persistent blocks through the counter `queue` (zero before the launch).
-hipError_t launch_pass1_batch(int sample_bytes, uint32_t window, const Pass1Args* stacks, const BatchItem* items,
-                              int n_items, int* queue, int blocks, hipStream_t s);
+// `lens`: some frame of the launch is remapped by the lens row walk.
+hipError_t launch_pass1_batch(int sample_bytes, uint32_t window, bool lens, const Pass1Args* stacks,
+                              const BatchItem* items, int n_items, int* queue, int blocks, hipStream_t s);
 // One pyramid level of many stacks: item b on block b, tiles of tw x th (64x16 or 16x16).
 hipError_t launch_pass2_batch(int sample_bytes, bool level0, int tw, int th, const Pass2Job* jobs,
                               const BatchItem* items, int n_items, hipStream_t s);

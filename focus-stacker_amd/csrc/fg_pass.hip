@@ -51,10 +51,22 @@ struct Pack<uint16_t> {
 
 // RGBA of ROI pixel (x, y) of a frame: remapped source (remapRigidImage) or aligned frame
 // (alpha = full support when the frame has none, which leaves applySupportInto a no-op).
-template <typename T>
+// LENS: the launch holds frames of the lens row walk, whose exact remap (the walk redone where the
+// direct products land near a rounding tie) is compiled in only then: its registers would otherwise
+// count against every pass-1 launch through the out-of-line exact path below.
+template <typename T, bool LENS>
 __device__ __forceinline__ void frame_rgba(const FrameDesc& f, int x, int y, uint32_t o[4]) {
     if (f.kind == 0) {
-        remap_sample<T, 3>(f.p, (const T*)f.src, (int)f.sw, (int)f.sh, x, y, o);
+        if constexpr (LENS) {
+            remap_sample<T, 3>(f.p, (const T*)f.src, (int)f.sw, (int)f.sh, x, y, o);
+        } else {
+            double sx, sy;
+            if (map_pixel(f.p, x, y, &sx, &sy)) {
+                sample_bilinear<T, 3>((const T*)f.src, (int)f.sw, (int)f.sh, sx, sy, o);
+            } else {
+                o[0] = o[1] = o[2] = o[3] = 0;
+            }
+        }
     } else {
         const int st = 3 + f.extra;
         const T* q = (const T*)f.src + ((size_t)y * f.sw + x) * st;
@@ -66,10 +78,10 @@ __device__ __forceinline__ void frame_rgba(const FrameDesc& f, int x, int y, uin
 }
 
 // frame_rgba out of line, for the rare pixels the fast remap path hands back
-template <typename T>
+template <typename T, bool LENS>
 __device__ __attribute__((noinline)) uint64_t frame_rgba_exact(const FrameDesc& f, int x, int y) {
     uint32_t o[4];
-    frame_rgba<T>(f, x, y, o);
+    frame_rgba<T, LENS>(f, x, y, o);
     return (uint64_t)o[0] | ((uint64_t)o[1] << 16) | ((uint64_t)o[2] << 32) | ((uint64_t)o[3] << 48);
 }
 
@@ -615,7 +627,7 @@ __device__ __forceinline__ V lane_window_sum(V x, int b) {
 // written to the hard image at once); compiled apart so that the other launches carry none of it.
 // PACK: the u8 hybrid with its frames cached, whose LDS is packed (alpha bytes, union support as
 // row lane masks) to keep four blocks per CU; likewise compiled apart.
-template <typename T, int RH, bool COL, bool PACK>
+template <typename T, int RH, bool COL, bool PACK, bool LENS>
 __device__ __forceinline__ void pass1_tile(const Pass1Args& a, const int bx, const int by) {
     using P = Pack<T>;
     using PT = typename P::type;
@@ -838,7 +850,7 @@ __device__ __forceinline__ void pass1_tile(const Pass1Args& a, const int bx, con
             if (redo & (1u << i)) {
                 redo &= ~(1u << i);
                 const int gy = Y0 - b + r;
-                const uint64_t e = frame_rgba_exact<T>(*fptr, gx, gy);
+                const uint64_t e = frame_rgba_exact<T, LENS>(*fptr, gx, gy);
                 const uint32_t o[4] = {(uint32_t)e & 0xffffu, (uint32_t)(e >> 16) & 0xffffu,
                                        (uint32_t)(e >> 32) & 0xffffu, (uint32_t)(e >> 48)};
                 emit(r, gy, o, std::false_type{});
@@ -1018,17 +1030,17 @@ __device__ __forceinline__ void xcd_tile(int* bx, int* by) {
     *by = st * kStrip + w % h;
 }
 
-template <typename T, int RH, bool COL, bool PACK>
+template <typename T, int RH, bool COL, bool PACK, bool LENS>
 __global__ void __launch_bounds__(256, 4) k_pass1(Pass1Args a) {
     int bx, by;
     xcd_tile(&bx, &by);
-    pass1_tile<T, RH, COL, PACK>(a, bx, by);
+    pass1_tile<T, RH, COL, PACK, LENS>(a, bx, by);
 }
 
 // Pass 1 of many stacks in one persistent launch: the blocks pull work items (a stack and one of its
 // tiles, every frame of that stack) from a queue in the order of `items` (longest first) until it
 // runs dry; every block reaches the end of the queue and returns.
-template <typename T, int RH, bool COL, bool PACK>
+template <typename T, int RH, bool COL, bool PACK, bool LENS>
 __global__ void __launch_bounds__(256, sizeof(T) == 1 ? 4 : 3)
     k_pass1_batch(const Pass1Args* __restrict__ stacks, const BatchItem* __restrict__ items, int n_items,
                   int* __restrict__ queue) {
@@ -1044,25 +1056,29 @@ __global__ void __launch_bounds__(256, sizeof(T) == 1 ? 4 : 3)
         const int bx = __builtin_amdgcn_readfirstlane((int)items[it].bx);
         const int by = __builtin_amdgcn_readfirstlane((int)items[it].by);
         const Pass1Args a = stacks[job];  // one scalar copy per item, as kernel arguments would be
-        pass1_tile<T, RH, COL, PACK>(a, bx, by);
+        pass1_tile<T, RH, COL, PACK, LENS>(a, bx, by);
         __syncthreads();  // the next item reuses the block's LDS
     }
 }
 
-template <typename T, int RH, bool COL, bool PACK>
-static hipError_t launch_pass1_k(const Pass1Args& a, hipStream_t s) {
+template <typename T, int RH, bool COL, bool PACK, bool LENS>
+static hipError_t launch_pass1_kl(const Pass1Args& a, hipStream_t s) {
     const size_t lds = pass1_lds_bytes((int)a.window, (int)sizeof(T), a.best != nullptr, pass1_slot_winners(a),
                                        a.norm_sum != nullptr, PACK);
     const int b = (int)a.window / 2, th = RH - 2 * b, tw = P1C - 2 * b;
     if (lds > 65536) {
-        hipError_t e =
-            hipFuncSetAttribute((const void*)k_pass1<T, RH, COL, PACK>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                (int)lds);
+        hipError_t e = hipFuncSetAttribute((const void*)k_pass1<T, RH, COL, PACK, LENS>,
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         if (e != hipSuccess) return e;
     }
     dim3 grid((a.w + tw - 1) / tw, (a.h + th - 1) / th);
-    hipLaunchKernelGGL((k_pass1<T, RH, COL, PACK>), grid, dim3(256), lds, s, a);
+    hipLaunchKernelGGL((k_pass1<T, RH, COL, PACK, LENS>), grid, dim3(256), lds, s, a);
     return hipGetLastError();
+}
+
+template <typename T, int RH, bool COL, bool PACK>
+static hipError_t launch_pass1_k(const Pass1Args& a, hipStream_t s) {
+    return a.lens ? launch_pass1_kl<T, RH, COL, PACK, true>(a, s) : launch_pass1_kl<T, RH, COL, PACK, false>(a, s);
 }
 
 // Hybrid winners of frames [k0, k1) from their cached weights (updateRange, src/fuse/blend.zig:
@@ -1147,10 +1163,10 @@ static int pass1_batch_occupancy(uint32_t window) {
     int per_cu = 0;
     const size_t lds = pass1_batch_lds((int)sizeof(T), window);
     if (lds > 65536)
-        (void)hipFuncSetAttribute((const void*)k_pass1_batch<T, RH, false, false>,
+        (void)hipFuncSetAttribute((const void*)k_pass1_batch<T, RH, false, false, false>,
                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)k_pass1_batch<T, RH, false, false>, 256,
-                                                     lds) != hipSuccess)
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)k_pass1_batch<T, RH, false, false, false>,
+                                                     256, lds) != hipSuccess)
         per_cu = 1;
     return per_cu > 0 ? per_cu : 1;
 }
@@ -1162,30 +1178,37 @@ int pass1_resident_blocks(int sb, int device) {
     return per_cu * cus;
 }
 
-template <typename T, int RH>
+template <typename T, int RH, bool LENS>
 static hipError_t launch_pass1_batch_t(uint32_t window, const Pass1Args* stacks, const BatchItem* items, int n_items,
                                        int* queue, int blocks, hipStream_t s) {
     const size_t lds = pass1_batch_lds((int)sizeof(T), window);
     if (lds > 65536) {
-        hipError_t e = hipFuncSetAttribute((const void*)k_pass1_batch<T, RH, false, false>,
+        hipError_t e = hipFuncSetAttribute((const void*)k_pass1_batch<T, RH, false, false, LENS>,
                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         if (e != hipSuccess) return e;
     }
-    hipLaunchKernelGGL((k_pass1_batch<T, RH, false, false>), dim3((unsigned)blocks), dim3(256), lds, s, stacks, items,
-                       n_items, queue);
+    hipLaunchKernelGGL((k_pass1_batch<T, RH, false, false, LENS>), dim3((unsigned)blocks), dim3(256), lds, s, stacks,
+                       items, n_items, queue);
     return hipGetLastError();
 }
 
-hipError_t launch_pass1_batch(int sb, uint32_t window, const Pass1Args* stacks, const BatchItem* items, int n_items,
-                              int* queue, int blocks, hipStream_t s) {
+template <typename T, int RH>
+static hipError_t launch_pass1_batch_l(bool lens, uint32_t window, const Pass1Args* stacks, const BatchItem* items,
+                                       int n_items, int* queue, int blocks, hipStream_t s) {
+    return lens ? launch_pass1_batch_t<T, RH, true>(window, stacks, items, n_items, queue, blocks, s)
+                : launch_pass1_batch_t<T, RH, false>(window, stacks, items, n_items, queue, blocks, s);
+}
+
+hipError_t launch_pass1_batch(int sb, uint32_t window, bool lens, const Pass1Args* stacks, const BatchItem* items,
+                              int n_items, int* queue, int blocks, hipStream_t s) {
     if (n_items <= 0) return hipSuccess;
     if (window < 3 || window > 31 || (window & 1) == 0 || blocks <= 0) return hipErrorInvalidValue;
     const bool big = pass1_rows((int)window) == 64;
     if (sb == 1)
-        return big ? launch_pass1_batch_t<uint8_t, 64>(window, stacks, items, n_items, queue, blocks, s)
-                   : launch_pass1_batch_t<uint8_t, 32>(window, stacks, items, n_items, queue, blocks, s);
-    return big ? launch_pass1_batch_t<uint16_t, 64>(window, stacks, items, n_items, queue, blocks, s)
-               : launch_pass1_batch_t<uint16_t, 32>(window, stacks, items, n_items, queue, blocks, s);
+        return big ? launch_pass1_batch_l<uint8_t, 64>(lens, window, stacks, items, n_items, queue, blocks, s)
+                   : launch_pass1_batch_l<uint8_t, 32>(lens, window, stacks, items, n_items, queue, blocks, s);
+    return big ? launch_pass1_batch_l<uint16_t, 64>(lens, window, stacks, items, n_items, queue, blocks, s)
+               : launch_pass1_batch_l<uint16_t, 32>(lens, window, stacks, items, n_items, queue, blocks, s);
 }
 
 hipError_t launch_pass1(const Pass1Args& a, hipStream_t s) {

@@ -129,6 +129,34 @@ def test_batch_replays_its_graph_with_other_frames(ctx, fg):
         assert np.array_equal(first[i], _alone(fg, ctx, stacks[i]))
 
 
+def test_batch_recaptures_when_lens_frames_come_and_go(ctx, fg):
+    """A batch run's pass-1 kernel carries the lens-path remap only when the run has lens frames, so a
+    captured schedule is captured again when that changes.  One geometry (same sizes, same ROI) runs
+    with homography poses, then with the same poses plus a centre shift and radial terms (the lens
+    path), and so on: runs 2-5 each flip the flag against the graph of the run before.  Every run
+    must equal the stacks fused alone under the same poses."""
+    stacks = _mix(fg, 3, 3, 6)
+    lensed = []
+    for i, s in enumerate(stacks):
+        t = dict(s)
+        t["poses"] = [fg.pose({f: getattr(p, f) for f, _ in fg.Pose._fields_}, center_shift_x=0.3 + 0.1 * k,
+                              center_shift_y=-0.2, radial_a=2e-4 * (i + 1)) for k, p in enumerate(s["poses"])]
+        lensed.append(t)
+    outs = [_out(fg, s) for s in stacks]
+    ref_h = [_alone(fg, ctx, s) for s in stacks]
+    ref_l = [_alone(fg, ctx, s) for s in lensed]
+    for i in range(len(stacks)):
+        assert not np.array_equal(ref_h[i], ref_l[i])
+    with fg.Batch(ctx) as b:
+        for run, (group, refs) in enumerate([(stacks, ref_h), (lensed, ref_l), (stacks, ref_h), (lensed, ref_l),
+                                             (stacks, ref_h)]):
+            status = b.fuse([_request(fg, s, o) for s, o in zip(group, outs)])
+            assert all(st["state"] == fg.REQ_DONE for st in status)
+            for i, o in enumerate(outs):
+                got = _host(o)
+                assert np.array_equal(got, refs[i]), f"run {run}, stack {i}: {int((got != refs[i]).sum())} differ"
+
+
 def test_batch_groups_methods_types_and_windows(ctx, fg):
     """A hybrid request (run through the plain calls), a 16-bit stack, a window-7 stack, a capped
     pyramid and lens poses, interleaved with 8-bit window-5 stacks: groups split where the class
