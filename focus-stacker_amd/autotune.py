@@ -18,7 +18,7 @@ only the time does.  The tuner measures, one knob at a time (the others at their
 
 It writes every measurement and the chosen values to focus-stacker_amd/tuning_mi355x.json and, with
 --write, the chosen values as the defaults in focus-stacker_amd/csrc/fg_tuning.h (rebuild the library
-afterwards).  Run it from the repository root on a GPU box:  python focus-stacker_amd/autotune.py
+afterwards).  Run it from the repository root on an MI355X host:  python focus-stacker_amd/autotune.py
 """
 import argparse
 import json
