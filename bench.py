@@ -50,8 +50,8 @@ def kernel_models(s: int) -> dict:
 # counters on a plain copy of known size (profiles/pmc_traffic_<workload>.md), keyed by
 # (frames, width, height, samples, method, levels)
 TRAFFIC = {
-    (143, 7952, 5304, "u16", "pyramid", 0): {"k_pass1": 130.449e9, "k_pass2_level0": 160.354e9},
-    (30, 7952, 5304, "u8", "pyramid", 0): {"k_pass1": 16.140e9, "k_pass2_level0": 20.357e9},
+    (143, 7952, 5304, "u16", "pyramid", 0): {"k_pass1": 130.303e9, "k_pass2_level0": 160.602e9},
+    (30, 7952, 5304, "u8", "pyramid", 0): {"k_pass1": 16.125e9, "k_pass2_level0": 20.356e9},
 }
 COPY_PROBE_BYTES = 256 << 20
 
