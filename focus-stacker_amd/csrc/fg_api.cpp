@@ -14,6 +14,7 @@
 #include "../../include/focus_gpu.h"
 #include "fg_host.h"
 #include "fg_internal.h"
+#include "fg_upload.h"
 
 namespace {
 
@@ -214,6 +215,8 @@ struct fg_stack {
     hipEvent_t up_ev[8];
     hipEvent_t free_ev[8];
     bool free_rec[8];
+    fg::HostUploader* up;  // pinned chunk ring for the uploads (nullptr: plain copies from the caller's memory)
+    bool up_tried;
     float* accDbg;
     uint8_t* out_dev;
     float* merge_gs;
@@ -233,6 +236,16 @@ struct fg_stack {
 namespace {
 
 constexpr uint32_t kStageSlots = 8;
+
+// host memory that HIP allocated page-locked (the DMA engine reads it in place)
+bool host_page_locked(const void* p) {
+    hipPointerAttribute_t at;
+    if (hipPointerGetAttributes(&at, p) != hipSuccess) {
+        (void)hipGetLastError();  // pageable memory: an error on some runtimes, type 0 on others
+        return false;
+    }
+    return at.type == hipMemoryTypeHost;
+}
 
 hipEvent_t take_event(fg_stack* s) {
     hipEvent_t e = nullptr;
@@ -312,6 +325,8 @@ void free_stack(fg_stack* s, bool sync = true) {
     if (!s) return;
     if (sync && s->cstream) (void)hipStreamSynchronize(s->cstream);
     if (sync && s->stream) (void)hipStreamSynchronize(s->stream);
+    delete s->up;  // waits for transfers still reading its chunks
+    s->up = nullptr;
     if (s->cstream) {
         for (int j = 0; j < 8; j++) {
             (void)hipEventDestroy(s->up_ev[j]);
@@ -420,15 +435,28 @@ int describe_frame(fg_stack* s, const fg_image_desc* d, const void* px, bool dev
         const uint32_t j = s->stage_used;
         uint8_t* slot = s->stage[j];
         if (s->free_rec[j]) HIP_TRY(hipStreamWaitEvent(s->cstream, s->free_ev[j], 0), "slot wait");
+        if (!s->up_tried) {
+            s->up_tried = true;
+            s->up = new (std::nothrow) fg::HostUploader();
+            if (s->up && !s->up->init()) {
+                delete s->up;
+                s->up = nullptr;
+            }
+        }
+        // frames in page-locked memory (hipHostMalloc'd by the caller) go to the device directly
+        const bool direct = !s->up || host_page_locked(px);
         {
             PhaseMark pm(s, FG_PHASE_INGEST, s->cstream);
-            HIP_TRY(hipMemcpyAsync(slot, px, nb, hipMemcpyHostToDevice, s->cstream), "upload frame");
+            if (direct)
+                HIP_TRY(hipMemcpyAsync(slot, px, nb, hipMemcpyHostToDevice, s->cstream), "upload frame");
+            else
+                HIP_TRY(s->up->upload(slot, px, nb, s->cstream), "upload frame");
         }
         HIP_TRY(hipEventRecord(s->up_ev[j], s->cstream), "upload event");
         HIP_TRY(hipStreamWaitEvent(s->stream, s->up_ev[j], 0), "upload wait");
-        // the caller may release px on return: wait for this upload (pass 1 of earlier frames
-        // keeps running on the compute stream meanwhile)
-        HIP_TRY(hipEventSynchronize(s->up_ev[j]), "upload");
+        // the caller may release px on return: the uploader has read it by then; a direct copy is
+        // waited for (pass 1 of earlier frames keeps running on the compute stream meanwhile)
+        if (direct) HIP_TRY(hipEventSynchronize(s->up_ev[j]), "upload");
         f->src = slot;
     }
     if (!s->have_src) {

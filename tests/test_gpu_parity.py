@@ -697,3 +697,42 @@ def test_pinned_host_frames_match_pageable(ctx, fg, oracle):
     for p in pinned:
         p.close()
     assert np.array_equal(out1, out2)
+
+
+def test_host_frames_in_many_chunks_match_device_frames(ctx, fg):
+    """Host-fed frames larger than one 16 MiB upload chunk (3000x2000 u16: 36 MB, three chunks, the
+    last one partial), more frames than upload slots, from pageable memory (threaded copies through
+    the library's pinned chunks) and from page-locked memory (copied directly): the same image as
+    the same frames fed from device memory."""
+    import torch
+    import synth
+    spec = synth.config_spec(0, stack_id=77, frames=10, width=3000, height=2000, shift=3.0 * 3000 / 1024.0,
+                             dtype="u16")
+    st = synth.Stack(spec, device=torch.device("cuda:0"))
+    dev, host = [], []
+    for k in range(spec.frames):
+        f = st.frame(k)
+        host.append(f.cpu().numpy().astype(np.uint16))
+        dev.append((f - (f > 32767).to(torch.int32) * 65536).to(torch.int16).contiguous())
+    poses = [fg.pose(p) for p in st.poses]
+    roi = fg.common_overlap_roi([(spec.width, spec.height)] * spec.frames, poses)
+    w, h = roi[2] - roi[0], roi[3] - roi[1]
+    torch.cuda.synchronize()
+    s = fg.Stack(ctx, spec.frames, w, h, fg.SAMPLE_U16)
+    d = fg.ImageDesc(spec.width, spec.height, 3, 0, fg.SAMPLE_U16)
+    for f, p in zip(dev, poses):
+        s.add_frame_device(f.data_ptr(), d, p, roi)
+    ref = s.finish()
+    s.close()
+    print("3000x2000 u16 x 10, roi", roi, flush=True)
+    _, out = _run_stack(fg, ctx, host, st.poses, roi)
+    assert np.array_equal(out, ref), f"pageable: {int((out != ref).sum())} samples differ"
+    pinned = []
+    for f in host:
+        pa = fg.PinnedArray(f.shape, f.dtype)
+        pa.array[...] = f
+        pinned.append(pa)
+    _, out2 = _run_stack(fg, ctx, [p.array for p in pinned], st.poses, roi)
+    for p in pinned:
+        p.close()
+    assert np.array_equal(out2, ref), f"page-locked: {int((out2 != ref).sum())} samples differ"
