@@ -168,6 +168,10 @@ struct fg_context {
     void* remap_in;
     void* remap_out;
     size_t remap_in_cap, remap_out_cap;
+    // threads that copy host-fed frames into the stacks' page-locked upload chunks, shared by the
+    // context's stacks; made at the first such frame
+    std::mutex pool_mu;
+    fg::CopyPool* copy_pool;
 };
 
 // One stack: the pass-1 / pass-2 sequence.  Pass 1 runs over groups of frames (all pending frames
@@ -437,8 +441,14 @@ int describe_frame(fg_stack* s, const fg_image_desc* d, const void* px, bool dev
         if (s->free_rec[j]) HIP_TRY(hipStreamWaitEvent(s->cstream, s->free_ev[j], 0), "slot wait");
         if (!s->up_tried) {
             s->up_tried = true;
+            fg::CopyPool* pool = nullptr;
+            {
+                std::lock_guard<std::mutex> lk(s->ctx->pool_mu);
+                if (!s->ctx->copy_pool) s->ctx->copy_pool = new (std::nothrow) fg::CopyPool(fg::copy_pool_threads());
+                pool = s->ctx->copy_pool;
+            }
             s->up = new (std::nothrow) fg::HostUploader();
-            if (s->up && !s->up->init()) {
+            if (s->up && !s->up->init(pool)) {
                 delete s->up;
                 s->up = nullptr;
             }
@@ -654,6 +664,7 @@ void fg_context_destroy(fg_context* ctx) {
     (void)hipStreamDestroy(ctx->stream);
     if (ctx->remap_in) (void)hipFree(ctx->remap_in);
     if (ctx->remap_out) (void)hipFree(ctx->remap_out);
+    delete ctx->copy_pool;
     delete ctx;
 }
 

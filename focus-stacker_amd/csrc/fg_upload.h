@@ -22,6 +22,7 @@
 namespace fg {
 
 // memcpy split over `parts` threads: the caller's thread copies part 0 and waits for the others.
+// Callers on several threads may share one pool: their jobs run one after another.
 class CopyPool {
   public:
     explicit CopyPool(int parts) : parts_(std::max(1, parts)) {
@@ -43,6 +44,7 @@ class CopyPool {
             memcpy(d, s, n);
             return;
         }
+        std::lock_guard<std::mutex> job(call_mu_);
         {
             std::lock_guard<std::mutex> lk(mu_);
             d_ = d;
@@ -84,6 +86,7 @@ class CopyPool {
 
     const int parts_;
     std::vector<std::thread> th_;
+    std::mutex call_mu_;  // one job at a time
     std::mutex mu_;
     std::condition_variable cv_, done_;
     uint64_t gen_ = 0;  // one job per generation; the next is posted only after every part is done
@@ -104,8 +107,11 @@ class HostUploader {
     HostUploader(const HostUploader&) = delete;
     HostUploader& operator=(const HostUploader&) = delete;
 
-    // Pinned chunks, their events and the copy threads; false (nothing held) if any is refused.
-    bool init() {
+    // Pinned chunks and their events, filled through `pool` (not owned); false (nothing held) if
+    // any is refused.
+    bool init(CopyPool* pool) {
+        if (!pool) return false;
+        pool_ = pool;
         for (int b = 0; b < kChunks; b++) {
             if (hipHostMalloc((void**)&pin_[b], kChunkBytes, hipHostMallocDefault) != hipSuccess ||
                 hipEventCreateWithFlags(&ev_[b], hipEventDisableTiming) != hipSuccess) {
@@ -113,12 +119,6 @@ class HostUploader {
                 release();
                 return false;
             }
-        }
-        const unsigned hw = std::thread::hardware_concurrency();
-        pool_ = new (std::nothrow) CopyPool((int)std::min(8u, std::max(1u, hw / 2)));
-        if (!pool_) {
-            release();
-            return false;
         }
         return true;
     }
@@ -157,7 +157,6 @@ class HostUploader {
             ev_[b] = nullptr;
             pin_[b] = nullptr;
         }
-        delete pool_;
         pool_ = nullptr;
     }
 
@@ -167,5 +166,11 @@ class HostUploader {
     int next_ = 0;
     CopyPool* pool_ = nullptr;
 };
+
+// Threads for a context's copy pool: up to 8, half the host's hardware threads.
+inline int copy_pool_threads() {
+    const unsigned hw = std::thread::hardware_concurrency();
+    return (int)std::min(8u, std::max(1u, hw / 2));
+}
 
 }  // namespace fg

@@ -123,6 +123,8 @@ const char* fg_last_error(void);
 const char* fg_version(void);
 
 int fg_context_create(const fg_context_opts* opts, fg_context** out);
+/* Destroy the context's stacks, batches and servers first: they use its device, stream and the
+ * threads that copy host-fed frames. */
 void fg_context_destroy(fg_context* ctx);
 
 /* remap.remapRigidImage (src/remap.zig:166-204): remap `src` under `pose` into `roi`

@@ -2,7 +2,8 @@
 // upload chunks of host-fed frames: every byte lands once and in place for sizes that do not divide
 // by the thread count or the 4 KiB piece alignment, for copies below the threading threshold, and
 // for thousands of jobs in a row through one pool (each job is posted only after the previous one
-// has finished on every thread).  Host code only; prints the first mismatches and exits 1 on any.
+// has finished on every thread), and two callers sharing one pool.  Host code only; prints the
+// first mismatches and exits 1 on any.
 #include <stdio.h>
 #include <string.h>
 
@@ -37,6 +38,28 @@ int main() {
                 bad++;
             }
         }
+    }
+    {  // two callers sharing one pool (stacks of one context): their jobs run one after another
+        fg::CopyPool pool(4);
+        long tbad[2] = {0, 0};
+        auto caller = [&](int id) {
+            std::mt19937_64 r(100 + id);
+            const size_t n0 = (size_t)3 << 20;
+            std::vector<uint8_t> a(n0 + 8192), b(n0 + 8192);
+            for (int it = 0; it < 150; it++) {
+                const size_t n = n0 + r() % 8192;
+                for (size_t i = 0; i < n; i += 997) a[i] = (uint8_t)(it * 31 + id + i);
+                memset(b.data(), 0, b.size());
+                pool.copy(b.data(), a.data(), n);
+                if (memcmp(a.data(), b.data(), n) != 0) tbad[id]++;
+            }
+        };
+        std::thread t0(caller, 0), t1(caller, 1);
+        t0.join();
+        t1.join();
+        jobs += 300;
+        if (tbad[0] || tbad[1]) printf("mismatch: shared pool, %ld + %ld jobs\n", tbad[0], tbad[1]);
+        bad += tbad[0] + tbad[1];
     }
     printf("copy pool: %ld jobs, %ld mismatches\n", jobs, bad);
     return bad ? 1 : 0;
