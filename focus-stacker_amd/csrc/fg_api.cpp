@@ -241,6 +241,24 @@ namespace {
 
 constexpr uint32_t kStageSlots = 8;
 
+// The stack's page-locked chunk ring (and its context's copy threads), made at the first host-fed
+// frame or host-memory output; nullptr where they are refused (plain copies then).
+void ensure_uploader(fg_stack* s) {
+    if (s->up_tried) return;
+    s->up_tried = true;
+    fg::CopyPool* pool = nullptr;
+    {
+        std::lock_guard<std::mutex> lk(s->ctx->pool_mu);
+        if (!s->ctx->copy_pool) s->ctx->copy_pool = new (std::nothrow) fg::CopyPool(fg::copy_pool_threads());
+        pool = s->ctx->copy_pool;
+    }
+    s->up = new (std::nothrow) fg::HostUploader();
+    if (s->up && !s->up->init(pool)) {
+        delete s->up;
+        s->up = nullptr;
+    }
+}
+
 // host memory that HIP allocated page-locked (the DMA engine reads it in place)
 bool host_page_locked(const void* p) {
     hipPointerAttribute_t at;
@@ -439,20 +457,7 @@ int describe_frame(fg_stack* s, const fg_image_desc* d, const void* px, bool dev
         const uint32_t j = s->stage_used;
         uint8_t* slot = s->stage[j];
         if (s->free_rec[j]) HIP_TRY(hipStreamWaitEvent(s->cstream, s->free_ev[j], 0), "slot wait");
-        if (!s->up_tried) {
-            s->up_tried = true;
-            fg::CopyPool* pool = nullptr;
-            {
-                std::lock_guard<std::mutex> lk(s->ctx->pool_mu);
-                if (!s->ctx->copy_pool) s->ctx->copy_pool = new (std::nothrow) fg::CopyPool(fg::copy_pool_threads());
-                pool = s->ctx->copy_pool;
-            }
-            s->up = new (std::nothrow) fg::HostUploader();
-            if (s->up && !s->up->init(pool)) {
-                delete s->up;
-                s->up = nullptr;
-            }
-        }
+        ensure_uploader(s);
         // frames in page-locked memory (hipHostMalloc'd by the caller) go to the device directly
         const bool direct = !s->up || host_page_locked(px);
         {
@@ -1136,8 +1141,13 @@ int fg_stack_finish(fg_stack* s, void* out_px) {
     if (!out_px) return fail(FG_ERR_INVALID_ARG, "null output");
     int rc = finish(s);
     if (rc) return rc;
-    HIP_TRY(hipMemcpyAsync(out_px, s->out_dev, (size_t)s->W * s->H * 3 * s->sb, hipMemcpyDeviceToHost, s->stream),
-            "copy out");
+    const size_t bytes = (size_t)s->W * s->H * 3 * s->sb;
+    ensure_uploader(s);
+    if (s->up && !host_page_locked(out_px)) {  // pageable: through the chunks, every byte there on return
+        HIP_TRY(s->up->download(out_px, s->out_dev, bytes, s->stream), "copy out");
+        return FG_OK;
+    }
+    HIP_TRY(hipMemcpyAsync(out_px, s->out_dev, bytes, hipMemcpyDeviceToHost, s->stream), "copy out");
     HIP_TRY(hipStreamSynchronize(s->stream), "finish");
     return FG_OK;
 }

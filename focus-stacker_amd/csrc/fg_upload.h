@@ -1,4 +1,5 @@
-// Uploads of host-fed frames from pageable memory (the buffers a CPU decoder fills): the frame goes to
+// Uploads of host-fed frames from pageable memory (the buffers a CPU decoder fills), and the copy of a
+// stack's output back into pageable memory: the frame goes to
 // the device through a ring of pinned chunks.  Each chunk is filled by a few threads at once (one
 // CPU copy stream does not keep up with the DMA engine), and its DMA is queued as soon as it is
 // filled, so the copy of one chunk overlaps the transfer of the ones before it.  upload() returns
@@ -142,6 +143,42 @@ class HostUploader {
             if (e == hipSuccess) e = hipEventRecord(ev_[b], st);
             if (e != hipSuccess) return e;
             pending_[b] = true;
+        }
+        return hipSuccess;
+    }
+
+    // n bytes from device memory src (complete in stream order on `st`) to host memory dst, the other
+    // way through the same chunks: each chunk's transfer is queued ahead, then copied out by the
+    // threads once it has landed.  Every byte is in dst on return.
+    hipError_t download(void* dst, const void* src, size_t n, hipStream_t st) {
+        uint8_t* d = (uint8_t*)dst;
+        const uint8_t* s = (const uint8_t*)src;
+        const size_t nc = (n + kChunkBytes - 1) / kChunkBytes;
+        auto issue = [&](size_t c) -> hipError_t {
+            const int b = (int)(c % kChunks);
+            if (pending_[b]) {  // an upload's transfer still reading the chunk
+                const hipError_t e = hipEventSynchronize(ev_[b]);
+                if (e != hipSuccess) return e;
+                pending_[b] = false;
+            }
+            const size_t off = c * kChunkBytes, len = std::min(kChunkBytes, n - off);
+            hipError_t e = hipMemcpyAsync(pin_[b], s + off, len, hipMemcpyDeviceToHost, st);
+            if (e == hipSuccess) e = hipEventRecord(ev_[b], st);
+            if (e == hipSuccess) pending_[b] = true;
+            return e;
+        };
+        for (size_t c = 0; c < nc && c < (size_t)kChunks; c++) {
+            const hipError_t e = issue(c);
+            if (e != hipSuccess) return e;
+        }
+        for (size_t c = 0; c < nc; c++) {
+            const int b = (int)(c % kChunks);
+            hipError_t e = hipEventSynchronize(ev_[b]);
+            if (e != hipSuccess) return e;
+            pending_[b] = false;
+            const size_t off = c * kChunkBytes;
+            pool_->copy(d + off, pin_[b], std::min(kChunkBytes, n - off));
+            if (c + kChunks < nc && (e = issue(c + kChunks)) != hipSuccess) return e;
         }
         return hipSuccess;
     }
