@@ -172,6 +172,9 @@ struct fg_context {
     // context's stacks; made at the first such frame
     std::mutex pool_mu;
     fg::CopyPool* copy_pool;
+    // fg_remap's page-locked chunk ring for pageable sources and outputs (under remap_mu)
+    fg::HostUploader* remap_up;
+    bool remap_up_tried;
 };
 
 // One stack: the pass-1 / pass-2 sequence.  Pass 1 runs over groups of frames (all pending frames
@@ -243,20 +246,25 @@ constexpr uint32_t kStageSlots = 8;
 
 // The stack's page-locked chunk ring (and its context's copy threads), made at the first host-fed
 // frame or host-memory output; nullptr where they are refused (plain copies then).
+fg::HostUploader* make_uploader(fg_context* c) {
+    fg::CopyPool* pool = nullptr;
+    {
+        std::lock_guard<std::mutex> lk(c->pool_mu);
+        if (!c->copy_pool) c->copy_pool = new (std::nothrow) fg::CopyPool(fg::copy_pool_threads());
+        pool = c->copy_pool;
+    }
+    fg::HostUploader* up = new (std::nothrow) fg::HostUploader();
+    if (up && !up->init(pool)) {
+        delete up;
+        up = nullptr;
+    }
+    return up;
+}
+
 void ensure_uploader(fg_stack* s) {
     if (s->up_tried) return;
     s->up_tried = true;
-    fg::CopyPool* pool = nullptr;
-    {
-        std::lock_guard<std::mutex> lk(s->ctx->pool_mu);
-        if (!s->ctx->copy_pool) s->ctx->copy_pool = new (std::nothrow) fg::CopyPool(fg::copy_pool_threads());
-        pool = s->ctx->copy_pool;
-    }
-    s->up = new (std::nothrow) fg::HostUploader();
-    if (s->up && !s->up->init(pool)) {
-        delete s->up;
-        s->up = nullptr;
-    }
+    s->up = make_uploader(s->ctx);
 }
 
 // host memory that HIP allocated page-locked (the DMA engine reads it in place)
@@ -666,6 +674,7 @@ void fg_context_destroy(fg_context* ctx) {
     if (!ctx) return;
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
+    delete ctx->remap_up;
     (void)hipStreamDestroy(ctx->stream);
     if (ctx->remap_in) (void)hipFree(ctx->remap_in);
     if (ctx->remap_out) (void)hipFree(ctx->remap_out);
@@ -720,12 +729,24 @@ int fg_remap(fg_context* ctx, const fg_image_desc* src, const void* src_px, cons
     if (e == hipSuccess) e = grow(&ctx->remap_out, &ctx->remap_out_cap, out_b);
     if (e != hipSuccess) return fail_hip(e, "hipMalloc");
     rc = FG_OK;
-    e = hipMemcpyAsync(ctx->remap_in, src_px, in_b, hipMemcpyHostToDevice, ctx->stream);
+    // pageable buffers through the context's page-locked chunks, page-locked ones directly
+    if (!ctx->remap_up_tried) {
+        ctx->remap_up_tried = true;
+        ctx->remap_up = make_uploader(ctx);
+    }
+    fg::HostUploader* up = ctx->remap_up;
+    const bool in_direct = !up || host_page_locked(src_px), out_direct = !up || host_page_locked(dst_px);
+    e = in_direct ? hipMemcpyAsync(ctx->remap_in, src_px, in_b, hipMemcpyHostToDevice, ctx->stream)
+                  : up->upload(ctx->remap_in, src_px, in_b, ctx->stream);
     if (e == hipSuccess) {
         rc = fg_remap_device(ctx, src, ctx->remap_in, pose, &r, ctx->remap_out, ctx->stream);
         if (rc == FG_OK) {
-            e = hipMemcpyAsync(dst_px, ctx->remap_out, out_b, hipMemcpyDeviceToHost, ctx->stream);
-            if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+            if (out_direct) {
+                e = hipMemcpyAsync(dst_px, ctx->remap_out, out_b, hipMemcpyDeviceToHost, ctx->stream);
+                if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+            } else {
+                e = up->download(dst_px, ctx->remap_out, out_b, ctx->stream);
+            }
         }
     }
     if (rc) return rc;
